@@ -1,0 +1,379 @@
+"""MI355X (gfx950) closed-shell CCSD amplitude engine.
+
+Python access to libaqamd.so (the C ABI declared in include/aqamd.h) for
+tests, benchmarks and scripting.  There is no CPU fallback: importing the
+compute entry points without the built library raises, and every call that
+fails in the library raises AqError with the library's message.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaqamd.so")
+
+AQ_PREC_FP64, AQ_PREC_FP32, AQ_PREC_BF16X3 = 0, 1, 2
+PRECISIONS = {"fp64": AQ_PREC_FP64, "fp32": AQ_PREC_FP32, "bf16x3": AQ_PREC_BF16X3}
+
+
+class AqError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[aqamd error {code}] {msg}")
+        self.code = code
+
+
+class TimingRecord(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("launches", C.c_int64), ("total_ms", C.c_double),
+                ("flops", C.c_double), ("bytes", C.c_double)]
+
+
+class RHFCCSDConfig(C.Structure):
+    _fields_ = [("convergence", C.c_double), ("max_iterations", C.c_int), ("diis_order", C.c_int),
+                ("diis_start", C.c_int), ("diis_damping", C.c_double), ("precision", C.c_int),
+                ("refine_below", C.c_double), ("min_reduced_flops", C.c_double), ("fused_update", C.c_int)]
+
+
+ITER_CB = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)
+
+_P = C.c_void_p
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+
+# name -> argtypes (all return int status unless listed in _RESTYPE)
+_SIGS = {
+    "aq_last_error": [],
+    "aq_version": [],
+    "aq_ctx_create": [C.c_int, C.c_size_t, C.POINTER(_P)],
+    "aq_ctx_destroy": [_P],
+    "aq_ctx_sync": [_P],
+    "aq_ctx_set_precision": [_P, C.c_int, C.c_double],
+    "aq_ctx_get_precision": [_P, C.POINTER(C.c_int)],
+    "aq_ctx_memory": [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
+    "aq_ctx_trim": [_P],
+    "aq_ctx_enable_timing": [_P, C.c_int],
+    "aq_ctx_timing": [_P, C.POINTER(TimingRecord), C.c_int, C.POINTER(C.c_int)],
+    "aq_ctx_timing_reset": [_P],
+    "aq_tensor_create": [_P, C.c_int, _I64P, C.c_int, C.POINTER(_P)],
+    "aq_tensor_copy": [_P, _P, C.POINTER(_P)],
+    "aq_tensor_destroy": [_P],
+    "aq_tensor_ndim": [_P, C.POINTER(C.c_int)],
+    "aq_tensor_lengths": [_P, _I64P],
+    "aq_tensor_size": [_P, _I64P],
+    "aq_tensor_data": [_P, C.POINTER(_P)],
+    "aq_contract": [_P, C.c_double, _P, C.c_char_p, _P, C.c_char_p, C.c_double, _P, C.c_char_p],
+    "aq_sum": [_P, C.c_double, _P, C.c_char_p, C.c_double, _P, C.c_char_p],
+    "aq_scale": [_P, _P, C.c_double, C.c_char_p],
+    "aq_fill": [_P, _P, C.c_double, C.c_double],
+    "aq_dot": [_P, _P, C.c_char_p, _P, C.c_char_p, _DP],
+    "aq_weight": [_P, _P, C.c_int, C.POINTER(_DP), C.c_double],
+    "aq_norm": [_P, _P, C.c_int, _DP],
+    "aq_read": [_P, _P, C.c_int64, _I64P, _DP],
+    "aq_write": [_P, _P, C.c_int64, _I64P, _DP, C.c_double, C.c_double],
+    "aq_read_all": [_P, _P, _DP],
+    "aq_write_all": [_P, _P, _DP],
+    "aq_slice": [_P, C.c_double, _P, _I64P, C.c_double, _P, _I64P, _I64P],
+    "aq_rhfccsd_default_config": [C.POINTER(RHFCCSDConfig)],
+    "aq_rhfccsd_create": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(RHFCCSDConfig), C.POINTER(_P)],
+    "aq_rhfccsd_destroy": [_P],
+    "aq_rhfccsd_mp2": [_P, _DP],
+    "aq_rhfccsd_iterate": [_P, _DP, _DP],
+    "aq_rhfccsd_run": [_P, ITER_CB, C.c_void_p, _DP, _DP, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "aq_rhfccsd_tensor": [_P, C.c_int, C.POINTER(_P)],
+    "aq_rhfccsd_current_precision": [_P, C.POINTER(C.c_int)],
+    "aq_synth_host": [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _DP, _DP],
+    "aq_synth_default_sigma": [C.c_int, C.c_int],
+    "aq_synth_build": [_P, C.c_int, C.c_int, C.c_int, _DP, _DP, C.POINTER(_P)],
+}
+_RESTYPE = {"aq_last_error": C.c_char_p, "aq_version": C.c_int, "aq_synth_default_sigma": C.c_double}
+
+_lib = None
+
+
+def lib():
+    """Load libaqamd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m aquarius_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, C.c_int)
+        _lib = L
+    return _lib
+
+
+def _ck(rc):
+    if rc != 0:
+        raise AqError(rc, lib().aq_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    """aq_ctx: one device, one stream, one HBM budget."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.h = _P()
+        _ck(lib().aq_ctx_create(device, hbm_budget, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            _ck(lib().aq_ctx_destroy(self.h))
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _ck(lib().aq_ctx_sync(self.h))
+
+    def set_precision(self, prec, min_reduced_flops=0.0):
+        prec = PRECISIONS.get(prec, prec)
+        _ck(lib().aq_ctx_set_precision(self.h, int(prec), float(min_reduced_flops)))
+
+    def memory(self):
+        a, b = C.c_size_t(), C.c_size_t()
+        _ck(lib().aq_ctx_memory(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def trim(self):
+        _ck(lib().aq_ctx_trim(self.h))
+
+    def enable_timing(self, on=True):
+        _ck(lib().aq_ctx_enable_timing(self.h, 1 if on else 0))
+
+    def timing(self):
+        n = C.c_int()
+        _ck(lib().aq_ctx_timing(self.h, None, 0, C.byref(n)))
+        recs = (TimingRecord * max(n.value, 1))()
+        _ck(lib().aq_ctx_timing(self.h, recs, n.value, C.byref(n)))
+        return {r.name.decode(): {"launches": r.launches, "total_ms": r.total_ms, "flops": r.flops,
+                                  "bytes": r.bytes} for r in recs[:n.value]}
+
+    def timing_reset(self):
+        _ck(lib().aq_ctx_timing_reset(self.h))
+
+    # tensor factory
+    def tensor(self, shape, zero=True):
+        return Tensor(self, shape, zero)
+
+    def from_numpy(self, arr):
+        arr = np.asarray(arr, dtype=np.float64)
+        t = Tensor(self, arr.shape, zero=False)
+        t.write_all(arr)
+        return t
+
+    # operators (CTFTensor surface)
+    def contract(self, alpha, A, ia, B, ib, beta, Cc, ic):
+        _ck(lib().aq_contract(self.h, alpha, A.h, ia.encode(), B.h, ib.encode(), beta, Cc.h, ic.encode()))
+
+    def sum(self, alpha, A, ia, beta, B, ib):
+        _ck(lib().aq_sum(self.h, alpha, A.h, ia.encode(), beta, B.h, ib.encode()))
+
+    def scale(self, X, alpha, idx):
+        _ck(lib().aq_scale(self.h, X.h, alpha, idx.encode()))
+
+    def fill(self, X, alpha, beta):
+        _ck(lib().aq_fill(self.h, X.h, alpha, beta))
+
+    def dot(self, A, ia, B, ib):
+        r = C.c_double()
+        _ck(lib().aq_dot(self.h, A.h, ia.encode(), B.h, ib.encode(), C.byref(r)))
+        return r.value
+
+    def weight(self, X, ds, shift=0.0):
+        arrs = [np.ascontiguousarray(d, dtype=np.float64) for d in ds]
+        ptrs = (_DP * len(arrs))(*[_dp(a) for a in arrs])
+        _ck(lib().aq_weight(self.h, X.h, len(arrs), ptrs, shift))
+
+    def norm(self, X, p):
+        r = C.c_double()
+        _ck(lib().aq_norm(self.h, X.h, p, C.byref(r)))
+        return r.value
+
+    def slice(self, alpha, A, startA, beta, B, startB, extent):
+        a = np.asarray(startA, dtype=np.int64)
+        b = np.asarray(startB, dtype=np.int64)
+        e = np.asarray(extent, dtype=np.int64)
+        _ck(lib().aq_slice(self.h, alpha, A.h, a.ctypes.data_as(_I64P), beta, B.h, b.ctypes.data_as(_I64P),
+                           e.ctypes.data_as(_I64P)))
+
+
+class Tensor:
+    """Dense column-major FP64 device tensor (aq_tensor)."""
+
+    def __init__(self, ctx, shape, zero=True, handle=None, owned=True):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.owned = owned
+        if handle is not None:
+            self.h = handle
+            return
+        self.h = _P()
+        lens = np.asarray(self.shape, dtype=np.int64)
+        _ck(lib().aq_tensor_create(ctx.h, len(self.shape), lens.ctypes.data_as(_I64P), 1 if zero else 0,
+                                   C.byref(self.h)))
+
+    @property
+    def size(self):
+        n = 1
+        for s in self.shape:
+            n *= s
+        return n
+
+    def free(self):
+        if self.h and self.owned:
+            _ck(lib().aq_tensor_destroy(self.h))
+        self.h = _P()
+
+    def write_all(self, arr):
+        arr = np.asarray(arr, dtype=np.float64)
+        if arr.shape != self.shape:
+            raise ValueError(f"shape {arr.shape} != tensor shape {self.shape}")
+        flat = np.ascontiguousarray(arr.ravel(order="F"))
+        _ck(lib().aq_write_all(self.ctx.h, self.h, _dp(flat)))
+
+    def numpy(self):
+        flat = np.empty(self.size, dtype=np.float64)
+        _ck(lib().aq_read_all(self.ctx.h, self.h, _dp(flat)))
+        return flat.reshape(self.shape, order="F")
+
+    def read(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.int64)
+        vals = np.empty(len(keys))
+        _ck(lib().aq_read(self.ctx.h, self.h, len(keys), keys.ctypes.data_as(_I64P), _dp(vals)))
+        return vals
+
+    def write(self, keys, vals, alpha=1.0, beta=0.0):
+        keys = np.ascontiguousarray(keys, dtype=np.int64)
+        vals = np.ascontiguousarray(vals, dtype=np.float64)
+        _ck(lib().aq_write(self.ctx.h, self.h, len(keys), keys.ctypes.data_as(_I64P), _dp(vals), alpha, beta))
+
+
+def default_config(**over):
+    cfg = RHFCCSDConfig()
+    _ck(lib().aq_rhfccsd_default_config(C.byref(cfg)))
+    for k, v in over.items():
+        if k == "precision":
+            v = PRECISIONS.get(v, v)
+        setattr(cfg, k, v)
+    return cfg
+
+
+FOCK = ("fAI", "fAB", "fIJ")
+BLOCKS = ("VABCD", "VABCI", "VABIJ", "VAIBJ", "VAIJB", "VAIJK", "VIJKL")
+STATE = {"T1": 0, "T2": 1, "Z1": 2, "Z2": 3, "Tau": 4, "FAE": 10, "FMI": 11, "FME": 12, "WMNIJ": 13,
+         "WMNEJ": 14, "WAMIJ": 15, "WAMEI": 16, "WAMIE": 17}
+
+
+class Integrals:
+    """Fock blocks + Dirac blocks resident on the device."""
+
+    def __init__(self, ctx, tensors):
+        self.ctx = ctx
+        self.t = tensors   # dict name -> Tensor
+
+    @classmethod
+    def from_numpy(cls, ctx, f, V):
+        t = {k: ctx.from_numpy(f[k]) for k in FOCK}
+        t.update({k: ctx.from_numpy(V[k]) for k in BLOCKS})
+        return cls(ctx, t)
+
+    @classmethod
+    def synthetic(cls, ctx, seed, o, v, nQ=None, host=None, sigma=0.0):
+        """Seeded synthetic construction built on the device.  Returns the
+        integrals; `host` (eps, B) may be passed to reuse a host draw."""
+        nQ = 3 * (o + v) if nQ is None else nQ
+        eps, B = synth_host(seed, o, v, nQ, sigma) if host is None else host
+        out = (_P * 10)()
+        _ck(lib().aq_synth_build(ctx.h, o, v, nQ, _dp(eps), _dp(B), out))
+        shapes = [(v, o), (v, v), (o, o), (v, v, v, v), (v, v, v, o), (v, v, o, o), (v, o, v, o), (v, o, o, v),
+                  (v, o, o, o), (o, o, o, o)]
+        names = FOCK + BLOCKS
+        return cls(ctx, {n: Tensor(ctx, s, handle=_P(out[k])) for k, (n, s) in enumerate(zip(names, shapes))})
+
+    def free(self):
+        for t in self.t.values():
+            t.free()
+
+
+def synth_host(seed, o, v, nQ, sigma=0.0):
+    """(eps[o+v], B[nQ, o+v, o+v]) of the seeded synthetic construction
+    (sigma <= 0: the size-normalised default scale)."""
+    n = o + v
+    eps = np.empty(n)
+    B = np.empty((nQ, n, n))
+    _ck(lib().aq_synth_host(seed, o, v, nQ, float(sigma), _dp(eps), _dp(B)))
+    return eps, B
+
+
+def synth_default_sigma(v, nQ):
+    return lib().aq_synth_default_sigma(v, nQ)
+
+
+class RHFCCSD:
+    """aq_rhfccsd: the closed-shell CCSD solver (RHFCCSD, rhfccsd.cxx)."""
+
+    def __init__(self, ctx, ints, config=None, **over):
+        self.ctx = ctx
+        self.ints = ints
+        cfg = config if config is not None else default_config(**over)
+        f3 = (_P * 3)(*[ints.t[k].h for k in FOCK])
+        v7 = (_P * 7)(*[ints.t[k].h for k in BLOCKS])
+        self.h = _P()
+        _ck(lib().aq_rhfccsd_create(ctx.h, f3, v7, C.byref(cfg), C.byref(self.h)))
+        self.cfg = cfg
+        self.v, self.o = ints.t["fAI"].shape
+        self.history = []
+
+    def close(self):
+        if self.h:
+            _ck(lib().aq_rhfccsd_destroy(self.h))
+            self.h = _P()
+
+    @property
+    def mp2(self):
+        r = C.c_double()
+        _ck(lib().aq_rhfccsd_mp2(self.h, C.byref(r)))
+        return r.value
+
+    def iterate(self):
+        e, c = C.c_double(), C.c_double()
+        _ck(lib().aq_rhfccsd_iterate(self.h, C.byref(e), C.byref(c)))
+        return e.value, c.value
+
+    def run(self, callback=None):
+        hist = self.history
+
+        def _cb(it, e, c, dt, prec, user):
+            hist.append({"iter": it, "energy": e, "conv": c, "seconds": dt, "precision": prec})
+            if callback is not None:
+                callback(hist[-1])
+
+        cb = ITER_CB(_cb)
+        e, c, n, ok = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        _ck(lib().aq_rhfccsd_run(self.h, cb, None, C.byref(e), C.byref(c), C.byref(n), C.byref(ok)))
+        return {"energy": e.value, "conv": c.value, "iterations": n.value, "converged": bool(ok.value)}
+
+    def tensor(self, name):
+        p = _P()
+        _ck(lib().aq_rhfccsd_tensor(self.h, STATE[name], C.byref(p)))
+        n = C.c_int()
+        _ck(lib().aq_tensor_ndim(p, C.byref(n)))
+        lens = np.zeros(n.value, dtype=np.int64)
+        _ck(lib().aq_tensor_lengths(p, lens.ctypes.data_as(_I64P)))
+        return Tensor(self.ctx, tuple(lens), handle=p, owned=False)
+
+    def current_precision(self):
+        r = C.c_int()
+        _ck(lib().aq_rhfccsd_current_precision(self.h, C.byref(r)))
+        return r.value

@@ -1,0 +1,285 @@
+// Context: device, stream, HBM budget, scratch pool, operand cache, timing.
+// Replaces the role of Aquarius's Arena (src/util/distributed.hpp:28-74): the
+// place every tensor of one job lives.  One context = one HIP stream; nothing
+// here is global, so contexts on one device are independent.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace aqamd {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+size_t ctx_in_use(const aq_ctx* ctx) { return ctx->tensor_bytes + ctx->pool_bytes + ctx->cache_bytes; }
+
+static void pool_release_all(aq_ctx* ctx) {
+    for (auto& kv : ctx->pool_free) {
+        for (void* p : kv.second) {
+            (void)hipFree(p);
+            ctx->pool_bytes -= kv.first;
+        }
+    }
+    ctx->pool_free.clear();
+}
+
+void evict_cache(aq_ctx* ctx, size_t need) {
+    // least recently used first; copies pinned by the running operation stay
+    bool synced = false;
+    auto it = ctx->cache_lru.end();
+    while (it != ctx->cache_lru.begin() && ctx_in_use(ctx) + need > ctx->budget) {
+        --it;
+        auto ce = ctx->cache.find(*it);
+        if (ce == ctx->cache.end() || ce->second.pins > 0) continue;
+        if (!synced) {
+            AQ_HIP(hipStreamSynchronize(ctx->stream));
+            synced = true;
+        }
+        (void)hipFree(ce->second.ptr);
+        ctx->cache_bytes -= ce->second.bytes;
+        ctx->cache.erase(ce);
+        it = ctx->cache_lru.erase(it);
+    }
+}
+
+static void* raw_alloc(aq_ctx* ctx, size_t bytes) {
+    if (ctx_in_use(ctx) + bytes > ctx->budget) {
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        pool_release_all(ctx);
+        evict_cache(ctx, bytes);
+    }
+    AQ_CHECK(ctx_in_use(ctx) + bytes <= ctx->budget, AQ_ERR_OOM,
+             "HBM budget exceeded: need " + std::to_string(bytes) + " B, in use " +
+                 std::to_string(ctx_in_use(ctx)) + " of " + std::to_string(ctx->budget));
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        pool_release_all(ctx);
+        evict_cache(ctx, ctx->budget);   // drop every cached copy
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(AQ_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " B failed");
+        }
+    }
+    return p;
+}
+
+static size_t round_up(size_t b) {
+    const size_t g = b >= (1u << 20) ? (size_t(1) << 20) : 256;
+    return (b + g - 1) / g * g;
+}
+
+void* dev_alloc(aq_ctx* ctx, size_t bytes) {
+    bytes = round_up(bytes);
+    auto it = ctx->pool_free.find(bytes);
+    if (it != ctx->pool_free.end() && !it->second.empty()) {
+        void* p = it->second.back();
+        it->second.pop_back();
+        return p;
+    }
+    void* p = raw_alloc(ctx, bytes);
+    ctx->pool_bytes += bytes;
+    return p;
+}
+
+void dev_free(aq_ctx* ctx, void* p, size_t bytes) {
+    // stream-ordered reuse: later work on the same stream may reuse p
+    ctx->pool_free[round_up(bytes)].push_back(p);
+}
+
+double* tensor_alloc(aq_ctx* ctx, size_t bytes) {
+    bytes = std::max<size_t>(round_up(bytes), 256);
+    void* p = raw_alloc(ctx, bytes);
+    ctx->tensor_bytes += bytes;
+    return static_cast<double*>(p);
+}
+
+void tensor_free(aq_ctx* ctx, double* p, size_t bytes) {
+    bytes = std::max<size_t>(round_up(bytes), 256);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(p);
+    ctx->tensor_bytes -= bytes;
+}
+
+// ---------------------------------------------------------------- timing
+
+TimedLaunch::TimedLaunch(aq_ctx* c, const char* name, double flops, double bytes) : ctx(c), on(c->timing) {
+    if (!on) return;
+    rec.name = name;
+    rec.flops = flops;
+    rec.bytes = bytes;
+    auto take = [&]() {
+        if (!ctx->event_pool.empty()) {
+            hipEvent_t e = ctx->event_pool.back();
+            ctx->event_pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        AQ_HIP(hipEventCreate(&e));
+        return e;
+    };
+    rec.start = take();
+    rec.stop = take();
+    AQ_HIP(hipEventRecord(rec.start, ctx->stream));
+}
+
+TimedLaunch::~TimedLaunch() {
+    if (!on) return;
+    if (hipEventRecord(rec.stop, ctx->stream) == hipSuccess) ctx->pending.push_back(rec);
+}
+
+void resolve_timing(aq_ctx* ctx) {
+    for (auto& p : ctx->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, p.start, p.stop) == hipSuccess) {
+            auto& c = ctx->timing_classes[p.name];
+            c.launches += 1;
+            c.total_ms += ms;
+            c.flops += p.flops;
+            c.bytes += p.bytes;
+        }
+        ctx->event_pool.push_back(p.start);
+        ctx->event_pool.push_back(p.stop);
+    }
+    ctx->pending.clear();
+}
+
+}  // namespace aqamd
+
+using namespace aqamd;
+
+extern "C" {
+
+const char* aq_last_error(void) { return g_last_error.c_str(); }
+
+int aq_version(void) { return 100; }
+
+int aq_ctx_create(int device, size_t hbm_budget, aq_ctx** out) {
+    return guarded([&] {
+        AQ_CHECK(out, AQ_ERR_INVALID, "null output pointer");
+        int ndev = 0;
+        AQ_HIP(hipGetDeviceCount(&ndev));
+        AQ_CHECK(device >= 0 && device < ndev, AQ_ERR_INVALID, "no such HIP device");
+        AQ_HIP(hipSetDevice(device));
+        auto* ctx = new aq_ctx();
+        ctx->device = device;
+        try {
+            AQ_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+            size_t free_b = 0, total_b = 0;
+            AQ_HIP(hipMemGetInfo(&free_b, &total_b));
+            ctx->budget = hbm_budget ? hbm_budget : static_cast<size_t>(0.9 * double(free_b));
+            AQ_HIP(hipMalloc(&ctx->d_partials, sizeof(double) * (8 * 1024 + 64)));
+            AQ_HIP(hipHostMalloc(&ctx->h_result, sizeof(double) * 64, hipHostMallocDefault));
+        } catch (...) {
+            if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+            delete ctx;
+            throw;
+        }
+        *out = ctx;
+    });
+}
+
+int aq_ctx_destroy(aq_ctx* ctx) {
+    return guarded([&] {
+        if (!ctx) return;
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        resolve_timing(ctx);
+        pool_release_all(ctx);
+        for (auto& kv : ctx->cache) (void)hipFree(kv.second.ptr);
+        for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+        (void)hipFree(ctx->d_partials);
+        (void)hipHostFree(ctx->h_result);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+    });
+}
+
+int aq_ctx_sync(aq_ctx* ctx) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        resolve_timing(ctx);
+    });
+}
+
+int aq_ctx_set_precision(aq_ctx* ctx, int precision, double min_reduced_flops) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        AQ_CHECK(precision == AQ_PREC_FP64 || precision == AQ_PREC_FP32 || precision == AQ_PREC_BF16X3,
+                 AQ_ERR_INVALID, "unknown precision");
+        ctx->precision = precision;
+        ctx->min_reduced_flops = min_reduced_flops;
+    });
+}
+
+int aq_ctx_get_precision(const aq_ctx* ctx, int* precision) {
+    return guarded([&] {
+        AQ_CHECK(ctx && precision, AQ_ERR_INVALID, "null argument");
+        *precision = ctx->precision;
+    });
+}
+
+int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        if (in_use) *in_use = ctx_in_use(ctx);
+        if (budget) *budget = ctx->budget;
+    });
+}
+
+int aq_ctx_trim(aq_ctx* ctx) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        pool_release_all(ctx);
+        for (auto& kv : ctx->cache) (void)hipFree(kv.second.ptr);
+        ctx->cache.clear();
+        ctx->cache_lru.clear();
+        ctx->cache_bytes = 0;
+    });
+}
+
+int aq_ctx_enable_timing(aq_ctx* ctx, int enable) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        ctx->timing = enable != 0;
+    });
+}
+
+int aq_ctx_timing(aq_ctx* ctx, aq_timing_record* rec, int cap, int* n) {
+    return guarded([&] {
+        AQ_CHECK(ctx && n, AQ_ERR_INVALID, "null argument");
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        resolve_timing(ctx);
+        int i = 0;
+        for (auto& kv : ctx->timing_classes) {
+            if (rec && i < cap) {
+                std::memset(rec[i].name, 0, sizeof(rec[i].name));
+                std::strncpy(rec[i].name, kv.first.c_str(), sizeof(rec[i].name) - 1);
+                rec[i].launches = kv.second.launches;
+                rec[i].total_ms = kv.second.total_ms;
+                rec[i].flops = kv.second.flops;
+                rec[i].bytes = kv.second.bytes;
+            }
+            ++i;
+        }
+        *n = i;
+    });
+}
+
+int aq_ctx_timing_reset(aq_ctx* ctx) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        resolve_timing(ctx);
+        ctx->timing_classes.clear();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,438 @@
+// MFMA GEMMs for gfx950 (CDNA4, 64-wide wavefronts).
+//
+//   C[m + n*ldc] = alpha * sum_k opA(m,k) * opB(k,n) + beta * C[m + n*ldc]
+//   opA(m,k) = transA ? A[k + m*lda] : A[m + k*lda]
+//   opB(k,n) = transB ? B[n + k*ldb] : B[k + n*ldb]
+// with a batch dimension (grid z) and optional split-K (partials in FP64,
+// reduced in a fixed order, so results do not depend on scheduling).
+//
+// FP64 path: v_mfma_f64_16x16x4f64.  FP32 path: v_mfma_f32_16x16x4f32.
+// Both share one template: 256 threads = 4 waves, block tile BM x BN, BK = 16,
+// operands staged through double-buffered LDS (one barrier per k-tile), each
+// wave owning a (BM/WM) x (BN/WN) sub-tile of 16x16 MFMA blocks.
+// The MFMA's 16x4 "src0" operand is fed from the B tile and its 4x16 "src1"
+// from the A tile, so the 16x16 result lands with consecutive lanes on
+// consecutive rows m of column-major C (coalesced epilogue).
+//
+// BF16x3 path: each FP64 operand is split into hi = bf16(x), lo = bf16(x-hi)
+// planes; A*B ~ Ahi*Bhi + Ahi*Blo + Alo*Bhi with v_mfma_f32_16x16x32_bf16 and
+// FP32 accumulation.  Its operands are always K-contiguous copies with a
+// leading dimension padded to a multiple of 8 (16-byte vector loads).
+//
+// Tile order: the 1-D block id is remapped so that the blocks that land on
+// one XCD (ids are dealt round-robin over the 8 XCDs) cover consecutive tiles
+// with n fastest, i.e. they share A row-panels in that XCD's L2.
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cstdio>
+#include <memory>
+
+namespace aqamd {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short i16x8 __attribute__((ext_vector_type(8)));
+
+struct GemmArgs {
+    int64_t M, N, K, kchunk;
+    int nsplit;
+    const void* A;
+    int64_t lda, strideA, planeA;
+    const void* B;
+    int64_t ldb, strideB, planeB;
+    double alpha, beta;
+    double* C;
+    int64_t ldc, strideC;
+    double* P;   // split-K partials [split][batch][M*N], or null
+    int batch;
+    int transA, transB;
+};
+
+__device__ __forceinline__ int64_t remap_tile(int64_t bid, int64_t nb) {
+    const int64_t full = (nb / 8) * 8;
+    if (bid >= full) return bid;
+    return (bid % 8) * (nb / 8) + bid / 8;
+}
+
+// Result layout of the 16x16 MFMAs (measured on gfx950): lane l, register r
+// holds D[row][l % 16] with
+//   f64 16x16x4:           row = l/16 + 4*r
+//   f32 16x16x4, bf16 x32: row = 4*(l/16) + r
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef f64x4 acc_t;
+    static __device__ __forceinline__ acc_t run(double x, double y, acc_t c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(x, y, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int r) { return lane / 16 + 4 * r; }
+};
+template <> struct Mfma<float> {
+    typedef f32x4 acc_t;
+    static __device__ __forceinline__ acc_t run(float x, float y, acc_t c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(x, y, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int drow(int lane, int r) { return 4 * (lane / 16) + r; }
+};
+
+template <typename T, int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    constexpr int BK = 16;
+    constexpr int PAD = 1;
+    constexpr int WTM = BM / WM, WTN = BN / WN;   // wave tile
+    constexpr int FM = WTM / 16, FN = WTN / 16;   // 16x16 blocks per wave
+    constexpr int NA = (BM * BK + 255) / 256, NB = (BN * BK + 255) / 256;
+    static_assert(WM * WN == 4, "4 waves");
+    typedef typename Mfma<T>::acc_t acc_t;
+
+    __shared__ T As[2][BK][BM + PAD];
+    __shared__ T Bs[2][BK][BN + PAD];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+
+    const int64_t tilesN = (g.N + BN - 1) / BN;
+    const int64_t tilesM = (g.M + BM - 1) / BM;
+    const int64_t t = remap_tile(blockIdx.x, tilesM * tilesN);
+    const int64_t m0 = (t / tilesN) * BM, n0 = (t % tilesN) * BN;
+    const int bz = blockIdx.z;
+    const int b = bz / g.nsplit, split = bz % g.nsplit;
+    const int64_t kbeg = int64_t(split) * g.kchunk;
+    const int64_t kend = min(g.K, kbeg + g.kchunk);
+
+    const T* A = static_cast<const T*>(g.A) + int64_t(b) * g.strideA;
+    const T* B = static_cast<const T*>(g.B) + int64_t(b) * g.strideB;
+
+    T ra[NA], rb[NB];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+            const int e = tid + 256 * r;
+            int mm, kk;
+            if (g.transA) { kk = e % BK; mm = e / BK; } else { mm = e % BM; kk = e / BM; }
+            const int64_t m = m0 + mm, k = k0 + kk;
+            T v = T(0);
+            if (e < BM * BK && m < g.M && k < kend)
+                v = g.transA ? A[k + m * g.lda] : A[m + k * g.lda];
+            ra[r] = v;
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + 256 * r;
+            int nn, kk;
+            if (g.transB) { nn = e % BN; kk = e / BN; } else { kk = e % BK; nn = e / BK; }
+            const int64_t n = n0 + nn, k = k0 + kk;
+            T v = T(0);
+            if (e < BN * BK && n < g.N && k < kend)
+                v = g.transB ? B[n + k * g.ldb] : B[k + n * g.ldb];
+            rb[r] = v;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+            const int e = tid + 256 * r;
+            if (e >= BM * BK) continue;
+            int mm, kk;
+            if (g.transA) { kk = e % BK; mm = e / BK; } else { mm = e % BM; kk = e / BM; }
+            As[buf][kk][mm] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + 256 * r;
+            if (e >= BN * BK) continue;
+            int nn, kk;
+            if (g.transB) { nn = e % BN; kk = e / BN; } else { kk = e % BK; nn = e / BK; }
+            Bs[buf][kk][nn] = rb[r];
+        }
+    };
+
+    acc_t acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    if (nk > 0) load(kbeg);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int buf = int(kt & 1);
+        store(buf);
+        __syncthreads();
+        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + lane / 16;
+            T x[FN], y[FM];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) x[j] = Bs[buf][kr][wn * WTN + j * 16 + lane % 16];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) y[i] = As[buf][kr][wm * WTM + i * 16 + lane % 16];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::run(x[j], y[i], acc[i][j]);
+        }
+    }
+
+    // epilogue: lane holds C[m = mb + lane%16][n = nb + drow(lane, r)]
+    double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
+    const int64_t ldc = g.P ? g.M : g.ldc;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int64_t m = m0 + wm * WTM + i * 16 + lane % 16;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t n = n0 + wn * WTN + j * 16 + Mfma<T>::drow(lane, r);
+                if (m < g.M && n < g.N) {
+                    double v = g.alpha * double(acc[i][j][r]);
+                    if (!g.P && g.beta != 0.0) v += g.beta * Cb[m + n * ldc];
+                    Cb[m + n * ldc] = v;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ BF16x3
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
+    constexpr int BK = 32;
+    constexpr int KP = BK + 8;                      // padded row (bf16 elements)
+    constexpr int WTM = BM / WM, WTN = BN / WN;
+    constexpr int FM = WTM / 16, FN = WTN / 16;
+    constexpr int VA = BM * BK / 8, VB = BN * BK / 8;   // 8-element vectors per plane
+    constexpr int NA = (VA + 255) / 256, NB = (VB + 255) / 256;
+    static_assert(WM * WN == 4, "4 waves");
+
+    __shared__ i16x8 dummy_align;                   // keeps LDS 16-byte aligned
+    __shared__ __attribute__((aligned(16))) unsigned short As[2][2][BM][KP];   // [buf][hi/lo][m][k]
+    __shared__ __attribute__((aligned(16))) unsigned short Bs[2][2][BN][KP];
+    (void)dummy_align;
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = wave % WM, wn = wave / WM;
+
+    const int64_t tilesN = (g.N + BN - 1) / BN;
+    const int64_t tilesM = (g.M + BM - 1) / BM;
+    const int64_t t = remap_tile(blockIdx.x, tilesM * tilesN);
+    const int64_t m0 = (t / tilesN) * BM, n0 = (t % tilesN) * BN;
+    const int bz = blockIdx.z;
+    const int b = bz / g.nsplit, split = bz % g.nsplit;
+    const int64_t kbeg = int64_t(split) * g.kchunk;
+    const int64_t kend = min(g.K, kbeg + g.kchunk);
+
+    // operands: A[k + m*lda] (K-contiguous), B[k + n*ldb] (K-contiguous)
+    const unsigned short* A = static_cast<const unsigned short*>(g.A) + int64_t(b) * g.strideA;
+    const unsigned short* B = static_cast<const unsigned short*>(g.B) + int64_t(b) * g.strideB;
+
+    i16x8 ra[2][NA], rb[2][NB];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+            const int e = tid + 256 * r;
+            const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
+            const int64_t m = m0 + mm, k = k0 + kk;
+            i16x8 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
+            if (e < VA && m < g.M && k < kend) {
+                h = *reinterpret_cast<const i16x8*>(A + k + m * g.lda);
+                l = *reinterpret_cast<const i16x8*>(A + g.planeA + k + m * g.lda);
+            }
+            ra[0][r] = h;
+            ra[1][r] = l;
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + 256 * r;
+            const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
+            const int64_t n = n0 + nn, k = k0 + kk;
+            i16x8 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
+            if (e < VB && n < g.N && k < kend) {
+                h = *reinterpret_cast<const i16x8*>(B + k + n * g.ldb);
+                l = *reinterpret_cast<const i16x8*>(B + g.planeB + k + n * g.ldb);
+            }
+            rb[0][r] = h;
+            rb[1][r] = l;
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+            const int e = tid + 256 * r;
+            if (e >= VA) continue;
+            const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
+            *reinterpret_cast<i16x8*>(&As[buf][0][mm][kk]) = ra[0][r];
+            *reinterpret_cast<i16x8*>(&As[buf][1][mm][kk]) = ra[1][r];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + 256 * r;
+            if (e >= VB) continue;
+            const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
+            *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][kk]) = rb[0][r];
+            *reinterpret_cast<i16x8*>(&Bs[buf][1][nn][kk]) = rb[1][r];
+        }
+    };
+
+    f32x4 acc[FM][FN];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    if (nk > 0) load(kbeg);
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int buf = int(kt & 1);
+        store(buf);
+        __syncthreads();
+        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+        // src0 (16 x 32): lane holds row lane%16, k = 8*(lane/16) .. +7  (from the B tile)
+        // src1 (32 x 16): lane holds col lane%16, k = 8*(lane/16) .. +7  (from the A tile)
+        const int kq = 8 * (lane / 16);
+        bf16x8 xh[FN], xl[FN], yh[FM], yl[FM];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int nn = wn * WTN + j * 16 + lane % 16;
+            i16x8 h = *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][kq]);
+            i16x8 l = *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][kq]);
+            xh[j] = __builtin_bit_cast(bf16x8, h);
+            xl[j] = __builtin_bit_cast(bf16x8, l);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            const int mm = wm * WTM + i * 16 + lane % 16;
+            i16x8 h = *reinterpret_cast<const i16x8*>(&As[buf][0][mm][kq]);
+            i16x8 l = *reinterpret_cast<const i16x8*>(&As[buf][1][mm][kq]);
+            yh[i] = __builtin_bit_cast(bf16x8, h);
+            yl[i] = __builtin_bit_cast(bf16x8, l);
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i], acc[i][j], 0, 0, 0);
+            }
+    }
+
+    double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
+    const int64_t ldc = g.P ? g.M : g.ldc;
+#pragma unroll
+    for (int i = 0; i < FM; ++i) {
+        const int64_t m = m0 + wm * WTM + i * 16 + lane % 16;
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t n = n0 + wn * WTN + j * 16 + 4 * (lane / 16) + r;
+                if (m < g.M && n < g.N) {
+                    double v = g.alpha * double(acc[i][j][r]);
+                    if (!g.P && g.beta != 0.0) v += g.beta * Cb[m + n * ldc];
+                    Cb[m + n * ldc] = v;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------- dispatch
+
+struct TileCfg {
+    int BM, BN;
+};
+
+template <typename T, int BM, int BN, int WM, int WN>
+static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
+    hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN>), grid, dim3(256), 0, ctx->stream, a);
+}
+
+template <int BM, int BN, int WM, int WN>
+static void run_cfg_bf16(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
+    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN>), grid, dim3(256), 0, ctx->stream, a);
+}
+
+void gemm(aq_ctx* ctx, const GemmDesc& d) {
+    AQ_CHECK(d.M > 0 && d.N > 0 && d.K > 0 && d.batch > 0, AQ_ERR_INVALID, "empty GEMM");
+    AQ_CHECK(d.batch < 65536, AQ_ERR_UNSUPPORTED, "GEMM batch too large");
+    if (d.fmt == FMT_BF16X2)
+        AQ_CHECK(d.transA && !d.transB && d.lda % 8 == 0 && d.ldb % 8 == 0 && d.planeA % 8 == 0 &&
+                     d.planeB % 8 == 0 && d.strideA % 8 == 0 && d.strideB % 8 == 0,
+                 AQ_ERR_INVALID, "BF16x3 GEMM needs K-contiguous operands with 8-aligned leading dimensions");
+
+    // tile configuration
+    int BM = 128, BN = 128;
+    if (d.N <= 16) { BM = 128; BN = 16; }
+    else if (d.M <= 16) { BM = 16; BN = 128; }
+    else if (((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch < 256) { BM = 64; BN = 64; }
+    const int64_t tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+    AQ_CHECK(tiles < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "GEMM grid too large");
+
+    // split-K when the output alone cannot fill the device
+    const int BK = d.fmt == FMT_BF16X2 ? 32 : 16;
+    int nsplit = 1;
+    const int64_t ktiles = (d.K + BK - 1) / BK;
+    const int64_t want = 1024;
+    if (tiles * d.batch < 512 && ktiles >= 16) {
+        int64_t s = (want + tiles * d.batch - 1) / (tiles * d.batch);
+        s = std::min<int64_t>(s, ktiles / 8);
+        s = std::min<int64_t>(s, 256);
+        nsplit = int(std::max<int64_t>(s, 1));
+        while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
+    }
+    int64_t kchunk = ((ktiles + nsplit - 1) / nsplit) * BK;
+    nsplit = int((d.K + kchunk - 1) / kchunk);
+
+    GemmArgs a;
+    a.M = d.M; a.N = d.N; a.K = d.K; a.kchunk = kchunk; a.nsplit = nsplit;
+    a.A = d.A; a.lda = d.lda; a.strideA = d.strideA; a.planeA = d.planeA;
+    a.B = d.B; a.ldb = d.ldb; a.strideB = d.strideB; a.planeB = d.planeB;
+    a.alpha = d.alpha; a.beta = d.beta;
+    a.C = d.C; a.ldc = d.ldc; a.strideC = d.strideC;
+    a.batch = int(d.batch);
+    a.transA = d.transA; a.transB = d.transB;
+    a.P = nullptr;
+
+    std::unique_ptr<Scratch> part;
+    if (nsplit > 1) {
+        part.reset(new Scratch(ctx, size_t(nsplit) * d.batch * d.M * d.N * 8));
+        a.P = part->as<double>();
+    }
+    dim3 grid(unsigned(tiles), 1, unsigned(d.batch * nsplit));
+    const double flops = 2.0 * double(d.M) * double(d.N) * double(d.K) * double(d.batch);
+    const double bytes = double(fmt_bytes(d.fmt)) * (double(d.M) * d.K + double(d.N) * d.K) * d.batch +
+                         8.0 * double(d.M) * d.N * d.batch * (d.beta != 0.0 ? 2 : 1);
+    char name[64];
+    snprintf(name, sizeof(name), "%s_%dx%d", d.tag ? d.tag : "gemm", BM, BN);
+    {
+        TimedLaunch tl(ctx, name, flops, bytes);
+        if (d.fmt == FMT_F64) {
+            if (BM == 128 && BN == 128) run_cfg<double, 128, 128, 2, 2>(ctx, a, grid);
+            else if (BM == 64) run_cfg<double, 64, 64, 2, 2>(ctx, a, grid);
+            else if (BN == 16) run_cfg<double, 128, 16, 4, 1>(ctx, a, grid);
+            else run_cfg<double, 16, 128, 1, 4>(ctx, a, grid);
+        } else if (d.fmt == FMT_F32) {
+            if (BM == 128 && BN == 128) run_cfg<float, 128, 128, 2, 2>(ctx, a, grid);
+            else if (BM == 64) run_cfg<float, 64, 64, 2, 2>(ctx, a, grid);
+            else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid);
+            else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid);
+        } else {
+            if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
+            else if (BM == 64) run_cfg_bf16<64, 64, 2, 2>(ctx, a, grid);
+            else if (BN == 16) run_cfg_bf16<128, 16, 4, 1>(ctx, a, grid);
+            else run_cfg_bf16<16, 128, 1, 4>(ctx, a, grid);
+        }
+        AQ_HIP(hipGetLastError());
+    }
+    if (nsplit > 1)
+        launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
+}
+
+}  // namespace aqamd

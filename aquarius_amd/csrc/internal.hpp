@@ -1,0 +1,223 @@
+// Internal runtime of the CCSD engine: context, tensors, scratch pool,
+// operand cache and kernel launch interfaces.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <list>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/aqamd.h"
+
+namespace aqamd {
+
+// ---------------------------------------------------------------- errors
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define AQ_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess)                                                          \
+            throw ::aqamd::Error(AQ_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define AQ_CHECK(cond, code, msg)                                                      \
+    do {                                                                               \
+        if (!(cond)) throw ::aqamd::Error((code), (msg));                              \
+    } while (0)
+
+// Run `body` translating exceptions into a status code + last-error message.
+template <class F>
+int guarded(F&& body) {
+    try {
+        body();
+        return AQ_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_last_error("host allocation failed");
+        return AQ_ERR_OOM;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return AQ_ERR_INVALID;
+    }
+}
+
+// ------------------------------------------------------------ data formats
+
+enum Fmt : int { FMT_F64 = 0, FMT_F32 = 1, FMT_BF16X2 = 2 };  // BF16X2 = hi plane + lo plane
+
+inline size_t fmt_bytes(Fmt f) { return f == FMT_F64 ? 8 : f == FMT_F32 ? 4 : 4; }
+
+// ------------------------------------------------------------ timing
+
+struct TimingClass {
+    int64_t launches = 0;
+    double total_ms = 0.0, flops = 0.0, bytes = 0.0;
+};
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+    double flops, bytes;
+};
+
+// ------------------------------------------------------------ context
+
+struct CacheEntry {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    uint64_t version = 0;
+    bool valid = false;
+    int pins = 0;             // > 0 while an operation is using the copy
+    std::list<std::string>::iterator lru;
+};
+
+}  // namespace aqamd
+
+struct aq_tensor {
+    aq_ctx* ctx = nullptr;
+    int ndim = 0;
+    std::vector<int64_t> len;
+    int64_t size = 1;
+    double* data = nullptr;
+    uint64_t id = 0;
+    uint64_t version = 0;   // bumped by every write; keys the operand cache
+};
+
+struct aq_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t budget = 0;
+    size_t tensor_bytes = 0;
+    int precision = AQ_PREC_FP64;
+    double min_reduced_flops = 0.0;
+    uint64_t next_id = 1;
+
+    // scratch pool: exact-size free lists
+    std::map<size_t, std::vector<void*>> pool_free;
+    size_t pool_bytes = 0;       // bytes held by the pool (free + in use)
+
+    // operand cache: key -> entry (permuted / converted copies of tensors)
+    std::unordered_map<std::string, aqamd::CacheEntry> cache;
+    std::list<std::string> cache_lru;   // front = most recent
+    size_t cache_bytes = 0;
+
+    // reductions
+    double* d_partials = nullptr;   // device: 8 slots x 1024 partials, then 64 result slots
+    double* h_result = nullptr;     // pinned host result slots (64)
+
+    // timing
+    bool timing = false;
+    std::vector<aqamd::PendingTiming> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, aqamd::TimingClass> timing_classes;
+};
+
+namespace aqamd {
+
+// ---- memory (context.cpp)
+void* dev_alloc(aq_ctx* ctx, size_t bytes);        // pooled scratch
+void dev_free(aq_ctx* ctx, void* p, size_t bytes);   // back to the pool
+double* tensor_alloc(aq_ctx* ctx, size_t bytes);     // persistent (counted against budget)
+void tensor_free(aq_ctx* ctx, double* p, size_t bytes);
+size_t ctx_in_use(const aq_ctx* ctx);
+void evict_cache(aq_ctx* ctx, size_t need);
+
+struct Scratch {  // RAII pooled device buffer
+    aq_ctx* ctx;
+    void* p;
+    size_t n;
+    Scratch(aq_ctx* c, size_t bytes) : ctx(c), p(bytes ? dev_alloc(c, bytes) : nullptr), n(bytes) {}
+    ~Scratch() { if (p) dev_free(ctx, p, n); }
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// ---- timing (context.cpp)
+struct TimedLaunch {
+    aq_ctx* ctx;
+    PendingTiming rec;
+    bool on;
+    TimedLaunch(aq_ctx* c, const char* name, double flops, double bytes);
+    ~TimedLaunch();
+};
+void resolve_timing(aq_ctx* ctx);   // after a synchronize
+
+// ---- strided views (ops.cpp)
+constexpr int MAXDIM = 8;
+
+struct StridedPlan {
+    // output iteration space (unique labels of the destination)
+    int nd = 0;
+    int64_t len[MAXDIM];
+    int64_t sdst[MAXDIM];   // destination stride per output dim
+    int64_t ssrc[MAXDIM];   // source stride per output dim (0 = broadcast)
+    // reduced (summed) source dims
+    int nr = 0;
+    int64_t rlen[MAXDIM];
+    int64_t rsrc[MAXDIM];
+    int64_t nout = 1;
+    int64_t nred = 1;
+};
+
+// ---- kernels (elementwise.hip)
+void launch_fill(aq_ctx* ctx, double* x, int64_t n, double alpha, double beta);
+void launch_strided_sum(aq_ctx* ctx, const StridedPlan& p, double alpha, const double* src,
+                        double beta, double* dst);
+void launch_strided_convert(aq_ctx* ctx, const StridedPlan& p, const double* src, void* dst, Fmt fmt,
+                            int64_t plane_stride);
+void launch_weight(aq_ctx* ctx, double* x, int ndim, const int64_t* len, const double* d_dev,
+                   const int64_t* d_off, double shift);
+// deterministic reductions into ctx->h_result[slot]; caller syncs
+void launch_dot(aq_ctx* ctx, const double* a, const double* b, int64_t n, int slot);
+void launch_norm(aq_ctx* ctx, const double* x, int64_t n, int p, int slot);
+double sync_result(aq_ctx* ctx, int slot);
+void launch_scatter_keys(aq_ctx* ctx, double* x, int64_t n, const int64_t* keys, const double* vals,
+                         double alpha, double beta);
+void launch_gather_keys(aq_ctx* ctx, const double* x, int64_t n, const int64_t* keys, double* vals);
+void launch_reduce_splitk(aq_ctx* ctx, const double* P, int nsplit, int64_t m, int64_t n,
+                          int64_t batch, double beta, double* C, int64_t ldc, int64_t strideC);
+
+// fused CCSD update (elementwise.hip)
+struct UpdateArgs {
+    int o, v;
+    double* Z1; double* Z2; double* T1; double* T2; double* Tau;
+    const double* VABIJSA; const double* fAI;
+    const double* dA; const double* dI;
+};
+void launch_ccsd_update(aq_ctx* ctx, const UpdateArgs& a, int slot_e1, int slot_e2, int slot_c1,
+                        int slot_c2);
+
+// DIIS helpers (elementwise.hip)
+void launch_multi_dot(aq_ctx* ctx, const double* x, const double* const* ys, int k, int64_t n,
+                      int first_slot);
+void launch_lincomb(aq_ctx* ctx, double* out, const double* const* xs, const double* c, int k,
+                    int64_t n);
+
+// ---- GEMM (gemm.hip)
+struct GemmDesc {
+    int64_t M, N, K, batch;
+    Fmt fmt;
+    const void* A; int64_t lda; bool transA; int64_t strideA; int64_t planeA;  // plane = lo offset for BF16X2
+    const void* B; int64_t ldb; bool transB; int64_t strideB; int64_t planeB;
+    double alpha, beta;
+    double* C; int64_t ldc; int64_t strideC;
+    const char* tag;   // timing class name
+};
+void gemm(aq_ctx* ctx, const GemmDesc& d);
+
+}  // namespace aqamd

@@ -1,0 +1,511 @@
+// Closed-shell CCSD amplitude engine: RHFCCSD setup and iterate, DIIS and the
+// fixed-point driver, issued through the tensor operator surface
+// (include/aqamd.hpp) exactly as the reference issues them through CTFTensor.
+//
+//   setup      src/cc/rhfccsd.cxx:45-98
+//   iterate    src/cc/rhfccsd.cxx:194-327
+//   DIIS       src/convergence/diis.hpp:36-227
+//   driver     src/util/iterative.hpp:126-188
+//   D          src/operator/denominator.hpp:21-89
+//
+// Beyond the reference: the amplitude update + energy + convergence can run
+// as one fused kernel pair, DIIS inner products and extrapolations are fused
+// multi-operand kernels over device-resident history, and the contraction
+// precision can be reduced until the residual drops below a threshold, after
+// which the remaining iterations run in FP64 (iterative refinement).
+#include "internal.hpp"
+
+#include "../../include/aqamd.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <limits>
+#include <memory>
+
+using aqamd::tensor::Tensor;
+using aqamd::tensor::scalar;
+
+namespace aqamd {
+
+static void bump_t(const Tensor& t) { t.handle()->version += 1; }
+static double* dptr(const Tensor& t) { return t.handle()->data; }
+static int64_t tsize(const Tensor& t) { return t.handle()->size; }
+
+// Solve the (n x n) symmetric system A x = b in place (row-major A), Gaussian
+// elimination with partial pivoting.  Returns 0, or k > 0 when the k-th pivot
+// is exactly zero (the dsysv 'info > 0' condition, diis.hpp:171-177).
+static int solve_small(int n, std::vector<double> A, std::vector<double>& b) {
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[i * n + k]) > std::fabs(A[p * n + k])) p = i;
+        if (A[p * n + k] == 0.0) return k + 1;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[k * n + j], A[p * n + j]);
+            std::swap(b[k], b[p]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[i * n + k] / A[k * n + k];
+            if (f == 0.0) continue;
+            for (int j = k; j < n; ++j) A[i * n + j] -= f * A[k * n + j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double s = b[k];
+        for (int j = k + 1; j < n; ++j) s -= A[k * n + j] * b[j];
+        b[k] = s / A[k * n + k];
+    }
+    return 0;
+}
+
+// Pulay DIIS over device-resident history (diis.hpp:36-227).
+class DIIS {
+  public:
+    DIIS(aq_ctx* ctx, int order, int start, double damping, int nx, int ndx)
+        : ctx_(ctx), nextrap_(order), start_(start), damping_(damping), nx_(nx), ndx_(ndx),
+          e_((order + 1) * (order + 1), 0.0), c_(order + 1, 0.0), old_x_(order), old_dx_(order) {}
+
+    bool used_eps_retry = false;
+
+    void extrapolate(const std::vector<Tensor*>& x, const std::vector<Tensor*>& dx) {
+        const int n = nextrap_, n1 = n + 1;
+        if (n <= 1) return;
+        AQ_CHECK(n <= 8, AQ_ERR_UNSUPPORTED, "DIIS order above 8 is not supported");
+        // slot k holds iteration n-k (diis.hpp:82-85)
+        std::rotate(old_x_.begin(), old_x_.end() - 1, old_x_.end());
+        std::rotate(old_dx_.begin(), old_dx_.end() - 1, old_dx_.end());
+        {
+            std::vector<double> r(e_.size());
+            for (int i = 0; i < n1; ++i)
+                for (int j = 0; j < n1; ++j) r[i * n1 + j] = e_[((i + n) % n1) * n1 + (j + n) % n1];
+            e_.swap(r);
+            std::rotate(c_.begin(), c_.end() - 1, c_.end());
+        }
+        store(old_x_[0], x);    // :92-105
+        store(old_dx_[0], dx);  // :107-120
+
+        // e[i][0] = <dx, old_dx[i]> for every stored slot (:122-135)
+        int nreal = 1;
+        while (nreal < n && !old_dx_[nreal].empty()) ++nreal;
+        std::vector<double> p(nreal, 0.0);
+        for (int j = 0; j < ndx_; ++j) {
+            std::vector<const double*> ys;
+            for (int i = 0; i < nreal; ++i) ys.push_back(dptr(old_dx_[i][j]));
+            launch_multi_dot(ctx_, dptr(*dx[j]), ys.data(), nreal, tsize(*dx[j]), 0);
+            AQ_HIP(hipStreamSynchronize(ctx_->stream));
+            for (int i = 0; i < nreal; ++i) p[i] += ctx_->h_result[i];
+        }
+        resolve_timing(ctx_);
+        E(0, 0) = p[0];
+        for (int i = 1; i < nreal; ++i) {
+            E(i, 0) = p[i];
+            E(0, i) = p[i];
+        }
+        for (int i = 0; i < nreal; ++i) {   // :141-149
+            E(i, nreal) = -1.0;
+            E(nreal, i) = -1.0;
+            c_[i] = 0.0;
+        }
+        E(nreal, nreal) = 0.0;
+        c_[nreal] = -1.0;
+
+        if (nreal == 1) return;   // :151
+
+        if (--start_ > 1) {       // :153-164
+            if (damping_ > 0.0)
+                for (int i = 0; i < nx_; ++i) {
+                    double* a = dptr(old_x_[0][i]);
+                    const double* b = dptr(old_x_[1][i]);
+                    const double* xs[2] = {a, b};
+                    const double cf[2] = {damping_ - 1.0, damping_};
+                    launch_lincomb(ctx_, a, xs, cf, 2, tsize(old_x_[0][i]));
+                    bump_t(old_x_[0][i]);
+                }
+            return;
+        }
+
+        const int m = nreal + 1;   // :166-188
+        auto sub = [&]() {
+            std::vector<double> A(m * m);
+            for (int i = 0; i < m; ++i)
+                for (int j = 0; j < m; ++j) A[i * m + j] = E(i, j);
+            return A;
+        };
+        std::vector<double> b(c_.begin(), c_.begin() + m);
+        std::vector<double> A = sub();
+        int info = solve_small(m, A, b);
+        used_eps_retry = false;
+        if (info > 0) {
+            const double eps = E(nreal - 1, nreal - 1) * std::numeric_limits<double>::epsilon();
+            A = sub();
+            for (int i = 0; i < nreal; ++i) A[i * m + i] += eps;
+            b.assign(c_.begin(), c_.begin() + m);
+            info = solve_small(m, A, b);
+            used_eps_retry = true;
+        }
+        AQ_CHECK(info == 0, AQ_ERR_NUMERIC, "DIIS: Info in hesv: " + std::to_string(info));
+        for (int i = 0; i < m; ++i) c_[i] = b[i];
+
+        // x = sum_i c_i old_x[i], dx = sum_i c_i old_dx[i] (:205-226)
+        for (int j = 0; j < ndx_; ++j) {
+            std::vector<const double*> xs;
+            for (int i = 0; i < nreal; ++i) xs.push_back(dptr(old_dx_[i][j]));
+            launch_lincomb(ctx_, dptr(*dx[j]), xs.data(), c_.data(), nreal, tsize(*dx[j]));
+            bump_t(*dx[j]);
+        }
+        for (int j = 0; j < nx_; ++j) {
+            std::vector<const double*> xs;
+            for (int i = 0; i < nreal; ++i) xs.push_back(dptr(old_x_[i][j]));
+            launch_lincomb(ctx_, dptr(*x[j]), xs.data(), c_.data(), nreal, tsize(*x[j]));
+            bump_t(*x[j]);
+        }
+    }
+
+    const std::vector<double>& coefficients() const { return c_; }
+
+  private:
+    double& E(int i, int j) { return e_[i * (nextrap_ + 1) + j]; }
+
+    void store(std::vector<Tensor>& slot, const std::vector<Tensor*>& src) {
+        if (slot.empty()) {
+            for (auto* t : src) slot.emplace_back(*t);   // deep copy
+        } else {
+            for (size_t i = 0; i < src.size(); ++i) {
+                AQ_HIP(hipMemcpyAsync(dptr(slot[i]), dptr(*src[i]), size_t(tsize(*src[i])) * 8,
+                                      hipMemcpyDeviceToDevice, ctx_->stream));
+                bump_t(slot[i]);
+            }
+        }
+    }
+
+    aq_ctx* ctx_;
+    int nextrap_, start_;
+    double damping_;
+    int nx_, ndx_;
+    std::vector<double> e_, c_;
+    std::vector<std::vector<Tensor>> old_x_, old_dx_;
+};
+
+}  // namespace aqamd
+
+using namespace aqamd;
+
+struct aq_rhfccsd {
+    aq_ctx* ctx = nullptr;
+    aq_rhfccsd_config cfg{};
+    int o = 0, v = 0;
+    Tensor fAI, fAB, fIJ;
+    Tensor VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL;
+    Tensor T1, T2, Z1, Z2, Tau, T2SA, TauSA, VABIJSA, VAIJBSA;
+    Tensor FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE;
+    std::vector<double> dA, dI;
+    Tensor d_dA, d_dI;
+    std::unique_ptr<DIIS> diis;
+    double mp2 = 0.0, energy = 0.0, conv = std::numeric_limits<double>::max();
+    int iter = 0;
+
+    int next_precision() const {
+        if (cfg.precision == AQ_PREC_FP64) return AQ_PREC_FP64;
+        if (cfg.refine_below > 0.0 && conv < cfg.refine_below) return AQ_PREC_FP64;
+        return cfg.precision;
+    }
+
+    // rhfccsd.cxx:45-98
+    void setup() {
+        const int64_t vo[2] = {v, o}, vvoo[4] = {v, v, o, o}, voov[4] = {v, o, o, v};
+        auto mk = [&](std::initializer_list<int64_t> l) { return Tensor(ctx, std::vector<int64_t>(l), true); };
+        (void)vo;
+        (void)vvoo;
+        (void)voov;
+        T1 = mk({v, o});
+        T2 = mk({v, v, o, o});
+        Z1 = mk({v, o});
+        Z2 = mk({v, v, o, o});
+        Tau = mk({v, v, o, o});
+        T2SA = mk({v, v, o, o});
+        TauSA = mk({v, v, o, o});
+        VABIJSA = mk({v, v, o, o});
+        VAIJBSA = mk({v, o, o, v});
+        FAE = mk({v, v});
+        FMI = mk({o, o});
+        FME = mk({o, v});
+        WMNIJ = mk({o, o, o, o});
+        WMNEJ = mk({o, o, v, o});
+        WAMIJ = mk({v, o, o, o});
+        WAMEI = mk({v, o, v, o});
+        WAMIE = mk({v, o, o, v});
+
+        // Denominator: dA = -f_aa, dI = +f_ii (denominator.hpp:48-67)
+        {
+            std::vector<int64_t> ka(v), ki(o);
+            for (int a = 0; a < v; ++a) ka[a] = int64_t(a) + int64_t(a) * v;
+            for (int i = 0; i < o; ++i) ki[i] = int64_t(i) + int64_t(i) * o;
+            dA.assign(v, 0.0);
+            dI.assign(o, 0.0);
+            tensor::check(aq_read(ctx, fAB.handle(), v, ka.data(), dA.data()));
+            tensor::check(aq_read(ctx, fIJ.handle(), o, ki.data(), dI.data()));
+            for (auto& x : dA) x = -x;
+            d_dA = mk({v});
+            d_dI = mk({o});
+            d_dA.setAllData(dA);
+            d_dI.setAllData(dI);
+        }
+
+        T1 = fAI;                                      // :79
+        T1.weight({&dA, &dI});                         // :80
+        T2 = VABIJ;                                    // :82
+        T2.weight({&dA, &dA, &dI, &dI});               // :83
+
+        Tau["abij"] = T2["abij"];                      // :85
+        Tau["abij"] += T1["ai"] * T1["bj"];            // :86
+
+        VABIJSA["abij"] = 2 * VABIJ["abij"];           // :88
+        VABIJSA["abij"] -= VABIJ["abji"];              // :89
+
+        VAIJBSA["aijb"] = 2 * VAIJB["aijb"];           // :91
+        VAIJBSA["aijb"] -= VAIBJ["aibj"];              // :92
+
+        mp2 = 2 * scalar(fAI["ai"] * T1["ai"]) + scalar(VABIJSA["abij"] * Tau["abij"]);   // :94
+        diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));   // :98
+    }
+
+    // rhfccsd.cxx:194-327
+    void iterate() {
+        const int prec = next_precision();
+        tensor::check(aq_ctx_set_precision(ctx, prec, cfg.min_reduced_flops));
+
+        Tau["abij"] = T2["abij"];                                  // :231
+        Tau["abij"] += T1["ai"] * T1["bj"];                        // :232
+        TauSA["abij"] = 2 * Tau["abij"];                           // :234
+        TauSA["abij"] -= Tau["abji"];                              // :235
+        T2SA["abij"] = 2 * T2["abij"];                             // :237
+        T2SA["abij"] -= T2["abji"];                                // :238
+
+        FME["me"] = fAI["em"];                                     // :244
+        FME["me"] += VABIJSA["efmn"] * T1["fn"];                   // :245
+
+        FMI["mi"] = fIJ["mi"];                                     // :247
+        FMI["mi"] += VABIJSA["efmn"] * T2["efin"];                 // :248
+        FMI["mi"] += FME["me"] * T1["ei"];                         // :249
+        FMI["mi"] += 2 * VAIJK["finm"] * T1["fn"];                 // :250
+        FMI["mi"] -= VAIJK["fimn"] * T1["fn"];                     // :251
+
+        FAE["ae"] = fAB["ae"];                                     // :253
+        FAE["ae"] -= VABIJSA["efmn"] * T2["afmn"];                 // :254
+        FAE["ae"] -= FME["me"] * T1["am"];                         // :255
+        FAE["ae"] += 2 * VABCI["efam"] * T1["fm"];                 // :256
+        FAE["ae"] -= VABCI["feam"] * T1["fm"];                     // :257
+
+        WMNIJ["mnij"] = 0.5 * VIJKL["mnij"];                       // :259
+        WMNIJ["mnij"] += 0.5 * VABIJ["efmn"] * Tau["efij"];        // :260
+        WMNIJ["mnij"] += VAIJK["ejmn"] * T1["ei"];                 // :261
+        WMNIJ["mnij"] += WMNIJ["nmji"];                            // :262 (aliased)
+
+        WMNEJ["mnej"] = VAIJK["ejmn"];                             // :264
+        WMNEJ["mnej"] += VABIJ["efmn"] * T1["fj"];                 // :265
+
+        WAMIJ["amij"] = VAIJK["amij"];                             // :267
+        WAMIJ["amij"] += VABCI["efam"] * Tau["efij"];              // :268
+        WAMIJ["amij"] += VAIBJ["amej"] * T1["ei"];                 // :269
+        WAMIJ["amij"] += VAIJB["amie"] * T1["ej"];                 // :270
+
+        WAMIE["amie"] = VAIJBSA["amie"];                           // :272
+        WAMIE["amie"] += 0.5 * VABIJSA["efmn"] * T2SA["afin"];     // :273
+        WAMIE["amie"] += 2 * VABCI["feam"] * T1["fi"];             // :274
+        WAMIE["amie"] -= VABCI["efam"] * T1["fi"];                 // :275
+        WAMIE["amie"] -= 2 * WMNEJ["mnei"] * T1["an"];             // :276
+        WAMIE["amie"] += WMNEJ["nmei"] * T1["an"];                 // :277
+
+        WAMEI["amei"] = VAIBJ["amei"];                             // :279
+        WAMEI["amei"] -= 0.5 * VABIJ["efnm"] * T2["afni"];         // :280
+        WAMEI["amei"] += VABCI["efam"] * T1["fi"];                 // :281
+        WAMEI["amei"] -= WMNEJ["nmei"] * T1["an"];                 // :282
+
+        Z1["ai"] = fAI["ai"];                                      // :290
+        Z1["ai"] += fAB["ae"] * T1["ei"];                          // :291
+        Z1["ai"] -= FMI["mi"] * T1["am"];                          // :292
+        Z1["ai"] += VAIJBSA["amie"] * T1["em"];                    // :293
+        Z1["ai"] += FME["me"] * T2SA["aeim"];                      // :294
+        Z1["ai"] += VABCI["efam"] * TauSA["efim"];                 // :295
+        Z1["ai"] -= WMNEJ["mnei"] * T2SA["eamn"];                  // :296
+
+        Z2["abij"] = 0.5 * VABIJ["abij"];                          // :298
+        Z2["abij"] += VABCI["abej"] * T1["ei"];                    // :299
+        Z2["abij"] -= WAMIJ["amij"] * T1["bm"];                    // :300
+        Z2["abij"] += FAE["ae"] * T2["ebij"];                      // :301
+        Z2["abij"] -= FMI["mi"] * T2["abmj"];                      // :302
+        Z2["abij"] += 0.5 * VABCD["abef"] * Tau["efij"];           // :303  particle-particle ladder
+        Z2["abij"] += 0.5 * WMNIJ["mnij"] * Tau["abmn"];           // :304
+        Z2["abij"] += 0.5 * WAMIE["amie"] * T2SA["ebmj"];          // :305  ring
+        Z2["abij"] -= 0.5 * WAMEI["amei"] * T2["ebjm"];            // :306  ring
+        Z2["abij"] -= WAMEI["amej"] * T2["ebim"];                  // :307  ring
+
+        Z2["abij"] += Z2["baji"];                                  // :309 (aliased)
+
+        // the residual, energy and convergence are FP64 whatever the contraction precision
+        tensor::check(aq_ctx_set_precision(ctx, AQ_PREC_FP64, 0.0));
+        if (cfg.fused_update) {
+            UpdateArgs u{o, v, dptr(Z1), dptr(Z2), dptr(T1), dptr(T2), dptr(Tau), dptr(VABIJSA), dptr(fAI),
+                         dptr(d_dA), dptr(d_dI)};
+            launch_ccsd_update(ctx, u, 0, 1, 2, 3);                // :313-323
+            AQ_HIP(hipStreamSynchronize(ctx->stream));
+            resolve_timing(ctx);
+            for (const Tensor* t : {&Z1, &Z2, &T1, &T2, &Tau}) bump_t(*t);
+            energy = 2 * ctx->h_result[0] + ctx->h_result[1];
+            conv = std::max(ctx->h_result[2], ctx->h_result[3]);
+        } else {
+            Z1.weight({&dA, &dI});                                 // :313
+            Z2.weight({&dA, &dA, &dI, &dI});                       // :314
+            T1 += Z1;                                              // :316
+            T2 += Z2;                                              // :317
+            Tau["abij"] = T2["abij"];                              // :319
+            Tau["abij"] += T1["ai"] * T1["bj"];                    // :320
+            energy = 2 * scalar(fAI["ai"] * T1["ai"]) + scalar(VABIJSA["abij"] * Tau["abij"]);   // :322
+            conv = std::max(Z1.norm(0), Z2.norm(0));               // :323
+        }
+        diis->extrapolate({&T1, &T2}, {&Z1, &Z2});                 // :325-326
+        ++iter;
+    }
+};
+
+extern "C" {
+
+int aq_rhfccsd_default_config(aq_rhfccsd_config* c) {
+    return guarded([&] {
+        AQ_CHECK(c, AQ_ERR_INVALID, "null config");
+        c->convergence = 1e-9;
+        c->max_iterations = 50;
+        c->diis_order = 5;
+        c->diis_start = 1;
+        c->diis_damping = 0.0;
+        c->precision = AQ_PREC_FP64;
+        c->refine_below = 1e-5;
+        c->min_reduced_flops = 1e8;
+        c->fused_update = 1;
+    });
+}
+
+int aq_rhfccsd_create(aq_ctx* ctx, const aq_tensor* const* f3, const aq_tensor* const* v7,
+                      const aq_rhfccsd_config* cfg, aq_rhfccsd** out) {
+    return guarded([&] {
+        AQ_CHECK(ctx && f3 && v7 && out, AQ_ERR_INVALID, "null argument");
+        for (int k = 0; k < 3; ++k) AQ_CHECK(f3[k], AQ_ERR_INVALID, "null Fock block");
+        for (int k = 0; k < 7; ++k) AQ_CHECK(v7[k], AQ_ERR_INVALID, "null integral block");
+        AQ_CHECK(f3[0]->ndim == 2, AQ_ERR_INVALID, "fAI must be [v,o]");
+        const int64_t v = f3[0]->len[0], o = f3[0]->len[1];
+        auto expect = [&](const aq_tensor* t, std::vector<int64_t> l, const char* nm) {
+            AQ_CHECK(t->len == l, AQ_ERR_INVALID, std::string("invalid ndim/lengths of ") + nm);
+        };
+        expect(f3[1], {v, v}, "fAB");
+        expect(f3[2], {o, o}, "fIJ");
+        expect(v7[0], {v, v, v, v}, "VABCD");
+        expect(v7[1], {v, v, v, o}, "VABCI");
+        expect(v7[2], {v, v, o, o}, "VABIJ");
+        expect(v7[3], {v, o, v, o}, "VAIBJ");
+        expect(v7[4], {v, o, o, v}, "VAIJB");
+        expect(v7[5], {v, o, o, o}, "VAIJK");
+        expect(v7[6], {o, o, o, o}, "VIJKL");
+        std::unique_ptr<aq_rhfccsd> cc(new aq_rhfccsd());
+        cc->ctx = ctx;
+        if (cfg) cc->cfg = *cfg;
+        else aq_rhfccsd_default_config(&cc->cfg);
+        AQ_CHECK(cc->cfg.diis_order >= 0 && cc->cfg.diis_order <= 8, AQ_ERR_UNSUPPORTED, "DIIS order must be 0..8");
+        cc->o = int(o);
+        cc->v = int(v);
+        auto b = [&](const aq_tensor* t) { return Tensor::borrow(ctx, const_cast<aq_tensor*>(t)); };
+        cc->fAI = b(f3[0]);
+        cc->fAB = b(f3[1]);
+        cc->fIJ = b(f3[2]);
+        cc->VABCD = b(v7[0]);
+        cc->VABCI = b(v7[1]);
+        cc->VABIJ = b(v7[2]);
+        cc->VAIBJ = b(v7[3]);
+        cc->VAIJB = b(v7[4]);
+        cc->VAIJK = b(v7[5]);
+        cc->VIJKL = b(v7[6]);
+        const int prev = ctx->precision;
+        const double prevf = ctx->min_reduced_flops;
+        ctx->precision = AQ_PREC_FP64;
+        cc->setup();
+        ctx->precision = prev;
+        ctx->min_reduced_flops = prevf;
+        *out = cc.release();
+    });
+}
+
+int aq_rhfccsd_destroy(aq_rhfccsd* cc) {
+    return guarded([&] { delete cc; });
+}
+
+int aq_rhfccsd_mp2(const aq_rhfccsd* cc, double* mp2) {
+    return guarded([&] {
+        AQ_CHECK(cc && mp2, AQ_ERR_INVALID, "null argument");
+        *mp2 = cc->mp2;
+    });
+}
+
+int aq_rhfccsd_iterate(aq_rhfccsd* cc, double* energy, double* conv) {
+    return guarded([&] {
+        AQ_CHECK(cc, AQ_ERR_INVALID, "null solver");
+        cc->iterate();
+        if (energy) *energy = cc->energy;
+        if (conv) *conv = cc->conv;
+    });
+}
+
+int aq_rhfccsd_run(aq_rhfccsd* cc, aq_iter_callback cb, void* user, double* energy, double* conv, int* iterations,
+                   int* converged) {
+    return guarded([&] {
+        AQ_CHECK(cc, AQ_ERR_INVALID, "null solver");
+        cc->conv = std::numeric_limits<double>::max();   // iterative.hpp:130
+        int it = 1;
+        for (; it <= cc->cfg.max_iterations && !(cc->conv < cc->cfg.convergence); ++it) {
+            const int prec = cc->next_precision();
+            auto t0 = std::chrono::steady_clock::now();
+            cc->iterate();
+            const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (cb) cb(it, cc->energy, cc->conv, dt, prec, user);
+        }
+        if (energy) *energy = cc->energy;
+        if (conv) *conv = cc->conv;
+        if (iterations) *iterations = it - 1;
+        if (converged) *converged = cc->conv < cc->cfg.convergence ? 1 : 0;
+    });
+}
+
+int aq_rhfccsd_tensor(const aq_rhfccsd* cc, int which, const aq_tensor** out) {
+    return guarded([&] {
+        AQ_CHECK(cc && out, AQ_ERR_INVALID, "null argument");
+        const Tensor* t = nullptr;
+        switch (which) {
+            case 0: t = &cc->T1; break;
+            case 1: t = &cc->T2; break;
+            case 2: t = &cc->Z1; break;
+            case 3: t = &cc->Z2; break;
+            case 4: t = &cc->Tau; break;
+            case 10: t = &cc->FAE; break;
+            case 11: t = &cc->FMI; break;
+            case 12: t = &cc->FME; break;
+            case 13: t = &cc->WMNIJ; break;
+            case 14: t = &cc->WMNEJ; break;
+            case 15: t = &cc->WAMIJ; break;
+            case 16: t = &cc->WAMEI; break;
+            case 17: t = &cc->WAMIE; break;
+            default: throw Error(AQ_ERR_INVALID, "unknown tensor selector");
+        }
+        *out = t->handle();
+    });
+}
+
+int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision) {
+    return guarded([&] {
+        AQ_CHECK(cc && precision, AQ_ERR_INVALID, "null argument");
+        *precision = cc->next_precision();
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,234 @@
+/*
+ * aqamd.h -- C ABI of the MI355X (gfx950) CCSD amplitude engine.
+ *
+ * The tensor entry points replace the operator surface that Aquarius's
+ * closed-shell CCSD issues through CTFTensor (src/tensor/ctf_tensor.hpp:16-285,
+ * src/tensor/ctf_tensor.cxx:19-516).  Tensors are dense, all-NS (no packed
+ * symmetry), double precision and column-major: the key of element
+ * (i0,i1,...) is sum_k i_k * prod_{j<k} len_j, exactly CTF's key convention
+ * (ctf_tensor.cxx:493-501).  Index strings use CTF/einsum label rules:
+ *   - a label in both inputs and not in the output is contracted,
+ *   - a label in one input only and not in the output is summed,
+ *   - a label in the output only is broadcast,
+ *   - a label repeated inside one tensor addresses its diagonal.
+ *
+ * Every function returns AQ_OK (0) or a positive error code; the message of
+ * the last failure on the calling thread is returned by aq_last_error().
+ * Nothing in the library aborts the process.
+ *
+ * Ordering: each context owns one HIP stream; operations are enqueued in call
+ * order and run asynchronously, except those documented as synchronising
+ * (aq_dot, aq_norm, aq_read*, aq_ctx_sync), which return host values.
+ * Contexts share no mutable state, so separate contexts may be driven from
+ * separate host threads.  One context must not be used from two threads at
+ * once.
+ */
+#ifndef AQAMD_H
+#define AQAMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aq_ctx aq_ctx;
+typedef struct aq_tensor aq_tensor;
+typedef struct aq_rhfccsd aq_rhfccsd;
+
+enum {
+    AQ_OK = 0,
+    AQ_ERR_INVALID = 1,      /* bad argument (ndim mismatch, bad index string, ...) */
+    AQ_ERR_HIP = 2,          /* HIP runtime failure */
+    AQ_ERR_OOM = 3,          /* device memory or the context's HBM budget exhausted */
+    AQ_ERR_UNSUPPORTED = 4,
+    AQ_ERR_NUMERIC = 5       /* e.g. singular DIIS system after the retry */
+};
+
+/* Contraction arithmetic of a context (see aq_ctx_set_precision). */
+enum {
+    AQ_PREC_FP64 = 0,        /* FP64 MFMA, FP64 accumulation */
+    AQ_PREC_FP32 = 1,        /* operands rounded to FP32, FP32 MFMA, FP32 accumulation */
+    AQ_PREC_BF16X3 = 2       /* operands split hi+lo BF16, three BF16 MFMA products, FP32 accumulation */
+};
+
+/* ---- errors / version ------------------------------------------------- */
+
+/* Message of the last failing call on this thread ("" if none).
+ * Reference: the TensorError family (tensor.hpp:834-898) and
+ * Logger::error (task.cxx:103-116), which abort there. */
+const char* aq_last_error(void);
+int aq_version(void);
+
+/* ---- context (Arena, src/util/distributed.hpp:28-74) ------------------- */
+
+/* Create a context on HIP device `device` with a stream of default priority.
+ * hbm_budget: bytes this context may hold in tensors and cached operand copies
+ * (0 = 90% of the device's free memory). */
+int aq_ctx_create(int device, size_t hbm_budget, aq_ctx** out);
+int aq_ctx_destroy(aq_ctx* ctx);
+/* Wait for all enqueued work of the context. */
+int aq_ctx_sync(aq_ctx* ctx);
+/* Arithmetic used by aq_contract.  Contractions with fewer than
+ * `min_reduced_flops` multiply-add FLOPs (2*M*N*K) keep FP64 even when a
+ * reduced precision is selected. */
+int aq_ctx_set_precision(aq_ctx* ctx, int precision, double min_reduced_flops);
+int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
+/* Bytes currently held (tensors + caches + pooled scratch) and the budget. */
+int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget);
+/* Drop cached operand copies and pooled scratch (keeps tensors). */
+int aq_ctx_trim(aq_ctx* ctx);
+/* Per-kernel-class timing with HIP events on the context's stream.
+ * When enabled every GEMM / elementwise launch is bracketed by events. */
+int aq_ctx_enable_timing(aq_ctx* ctx, int enable);
+/* Fill up to `cap` records; *n receives the number of classes.  Each record:
+ * name (NUL-terminated, <= 63 chars), launches, total ms, algorithmic flops,
+ * algorithmic bytes.  Synchronises. */
+typedef struct aq_timing_record {
+    char name[64];
+    int64_t launches;
+    double total_ms;
+    double flops;
+    double bytes;
+} aq_timing_record;
+int aq_ctx_timing(aq_ctx* ctx, aq_timing_record* rec, int cap, int* n);
+int aq_ctx_timing_reset(aq_ctx* ctx);
+
+/* ---- tensors ----------------------------------------------------------- */
+
+/* CTFTensor(name, arena, ndim, len, sym, zero) (ctf_tensor.cxx:104-120):
+ * dense all-NS tensor; zero != 0 clears it.  ndim may be 0 (a scalar). */
+int aq_tensor_create(aq_ctx* ctx, int ndim, const int64_t* len, int zero, aq_tensor** out);
+/* Deep copy (ctf_tensor.cxx:40-78). */
+int aq_tensor_copy(aq_ctx* ctx, const aq_tensor* src, aq_tensor** out);
+/* ~CTFTensor (ctf_tensor.cxx:123-127). */
+int aq_tensor_destroy(aq_tensor* t);
+int aq_tensor_ndim(const aq_tensor* t, int* ndim);
+int aq_tensor_lengths(const aq_tensor* t, int64_t* len);
+int aq_tensor_size(const aq_tensor* t, int64_t* size);
+/* Device address of the column-major data (for zero-copy interop). */
+int aq_tensor_data(const aq_tensor* t, double** dptr);
+
+/* ---- operators (CTFTensor methods) ------------------------------------- */
+
+/* C[idxC] = beta*C[idxC] + alpha * A[idxA]*B[idxB].
+ * CTFTensor::mult (ctf_tensor.cxx:434-442).  C may alias A or B. */
+int aq_contract(aq_ctx* ctx, double alpha, const aq_tensor* A, const char* idxA,
+                const aq_tensor* B, const char* idxB, double beta, aq_tensor* C, const char* idxC);
+
+/* B[idxB] = beta*B[idxB] + alpha*A[idxA].  CTFTensor::sum (ctf_tensor.cxx:454-458).
+ * Correct when A aliases B (the permuted source is read before B is written). */
+int aq_sum(aq_ctx* ctx, double alpha, const aq_tensor* A, const char* idxA,
+           double beta, aq_tensor* B, const char* idxB);
+
+/* X[idx] = alpha*X[idx].  CTFTensor::scale (ctf_tensor.cxx:461-464). */
+int aq_scale(aq_ctx* ctx, aq_tensor* X, double alpha, const char* idx);
+
+/* X = beta*X + alpha (scalar broadcast).  CTFTensor::sum(alpha, beta)
+ * (ctf_tensor.cxx:445-451). */
+int aq_fill(aq_ctx* ctx, aq_tensor* X, double alpha, double beta);
+
+/* *out = sum A[idxA]*B[idxB] over all labels.  CTFTensor::dot
+ * (ctf_tensor.cxx:467-478).  Synchronising. */
+int aq_dot(aq_ctx* ctx, const aq_tensor* A, const char* idxA,
+           const aq_tensor* B, const char* idxB, double* out);
+
+/* X[i0..] /= (d_0[i0] + d_1[i1] + ... + shift), or 0 where that |sum| < 1e-4.
+ * CTFTensor::weight (ctf_tensor.cxx:481-514).  d[k] points to len[k] host
+ * doubles. */
+int aq_weight(aq_ctx* ctx, aq_tensor* X, int ndim, const double* const* d, double shift);
+
+/* p = 0: max|x|, p = 1: sum|x|, p = 2: 2-norm.  CTFTensor::norm
+ * (ctf_tensor.cxx:403-431).  Synchronising. */
+int aq_norm(aq_ctx* ctx, const aq_tensor* X, int p, double* out);
+
+/* Key/value I/O with column-major keys (ctf_tensor.hpp:139-247).
+ * aq_read: getRemoteData; aq_write: writeRemoteData(alpha, beta, pairs)
+ * (x[key] = beta*x[key] + alpha*val).  aq_read_all / aq_write_all move the
+ * whole tensor in key order (getAllData).  aq_read* synchronise. */
+int aq_read(aq_ctx* ctx, const aq_tensor* X, int64_t n, const int64_t* keys, double* vals);
+int aq_write(aq_ctx* ctx, aq_tensor* X, int64_t n, const int64_t* keys, const double* vals,
+             double alpha, double beta);
+int aq_read_all(aq_ctx* ctx, const aq_tensor* X, double* host);
+int aq_write_all(aq_ctx* ctx, aq_tensor* X, const double* host);
+
+/* B[startB + r] = beta*B[startB + r] + alpha*A[startA + r] for r over the box
+ * `extent` (CTFTensor::slice, ctf_tensor.cxx:219-256, NS only). */
+int aq_slice(aq_ctx* ctx, double alpha, const aq_tensor* A, const int64_t* startA,
+             double beta, aq_tensor* B, const int64_t* startB, const int64_t* extent);
+
+/* ---- closed-shell CCSD engine (RHFCCSD, src/cc/rhfccsd.cxx) ------------ */
+
+/* Task options.  Defaults (aq_rhfccsd_default_config) follow the task spec
+ * (rhfccsd.cxx:332-353): convergence 1e-9, max_iterations 50, DIIS order 5,
+ * start 1, damping 0.  The precision fields have no reference counterpart. */
+typedef struct aq_rhfccsd_config {
+    double convergence;
+    int max_iterations;
+    int diis_order;
+    int diis_start;
+    double diis_damping;
+    /* contraction arithmetic while conv >= refine_below (AQ_PREC_*) */
+    int precision;
+    /* switch to FP64 contractions once conv < refine_below (0 = never) */
+    double refine_below;
+    /* contractions below this many FLOPs stay FP64 in reduced precision */
+    double min_reduced_flops;
+    /* 1: fuse weight + update + Tau + energy + conv into one kernel;
+     * 0: issue them as separate operator calls, as the reference does */
+    int fused_update;
+} aq_rhfccsd_config;
+
+int aq_rhfccsd_default_config(aq_rhfccsd_config* cfg);
+
+/* RHFCCSD::run setup (rhfccsd.cxx:45-98): takes the Fock blocks
+ * fAI[v,o], fAB[v,v], fIJ[o,o] and the Dirac blocks VABCD[v,v,v,v],
+ * VABCI[v,v,v,o], VABIJ[v,v,o,o], VAIBJ[v,o,v,o], VAIJB[v,o,o,v],
+ * VAIJK[v,o,o,o], VIJKL[o,o,o,o] (layout of rhfaomoints.cxx:58-65), builds the
+ * MP2 guess and the DIIS state.  The inputs are borrowed (not copied) and must
+ * outlive the solver and stay unmodified. */
+int aq_rhfccsd_create(aq_ctx* ctx, const aq_tensor* const* fock3, const aq_tensor* const* v7,
+                      const aq_rhfccsd_config* cfg, aq_rhfccsd** out);
+int aq_rhfccsd_destroy(aq_rhfccsd* cc);
+/* MP2 energy of the guess (rhfccsd.cxx:94). */
+int aq_rhfccsd_mp2(const aq_rhfccsd* cc, double* mp2);
+/* One RHFCCSD::iterate (rhfccsd.cxx:194-327): energy and conv as computed
+ * before the DIIS extrapolation. */
+int aq_rhfccsd_iterate(aq_rhfccsd* cc, double* energy, double* conv);
+/* Iterative::run (iterative.hpp:126-166): iterate until conv < convergence or
+ * max_iterations.  *converged = 0 reports non-convergence (the reference only
+ * logs it).  `log_cb`, if non-null, is called after every iteration. */
+typedef void (*aq_iter_callback)(int iter, double energy, double conv, double seconds, int precision, void* user);
+int aq_rhfccsd_run(aq_rhfccsd* cc, aq_iter_callback log_cb, void* user,
+                   double* energy, double* conv, int* iterations, int* converged);
+/* Amplitudes / state: which = 0: T1[v,o], 1: T2[v,v,o,o], 2: Z1, 3: Z2,
+ * 4: Tau; 10..17: FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE. */
+int aq_rhfccsd_tensor(const aq_rhfccsd* cc, int which, const aq_tensor** out);
+/* Precision the next iteration's contractions will use. */
+int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision);
+
+/* ---- synthetic closed-shell integrals (workload construction) ---------- */
+
+/* Host: orbital energies (eps[o+v], occupied sorted then virtual sorted) and
+ * symmetric three-index factors B[Q][p][q] (nQ*(o+v)^2 doubles) of the seeded
+ * construction: eps_occ ~ U[-1.5,-0.3], eps_vir ~ U[0.2,5.0],
+ * B^Q_pq = B^Q_qp ~ sigma*N(0,1), counter-based splitmix64 + Box-Muller.
+ * sigma <= 0 selects the size-normalised default
+ * aq_synth_default_sigma(v, nQ) = 0.05*sqrt(150/(v*sqrt(nQ))), which keeps the
+ * effective CC coupling v*sigma^2*sqrt(nQ) of the v=18, nQ=72 system at every
+ * size (sigma = 0.05 at any size makes CCSD diverge once v*sqrt(nQ) >> 150). */
+int aq_synth_host(uint64_t seed, int o, int v, int nQ, double sigma, double* eps, double* B);
+double aq_synth_default_sigma(int v, int nQ);
+/* Device: build f blocks (fAI = 0, fAB/fIJ diagonal) and the seven Dirac
+ * blocks <pq|rs> = (pr|qs), (pq|rs) = sum_Q B^Q_pq B^Q_rs, from host eps and B
+ * as produced by aq_synth_host.  Creates 10 tensors: out[0..2] = fAI, fAB, fIJ;
+ * out[3..9] = VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL. */
+int aq_synth_build(aq_ctx* ctx, int o, int v, int nQ, const double* eps, const double* B,
+                   aq_tensor** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AQAMD_H */

@@ -4,8 +4,11 @@ Construction (the synthetic CCSD workloads o=20/v=140, o=48/v=288 and the
 serving stream):
   * occupied orbital energies ~ U[-1.5,-0.3] Eh, virtual ~ U[0.2,5.0] Eh,
     each sorted ascending; Fock f = diag(eps), so f_ai = 0;
-  * three-index factors B^Q_pq = B^Q_qp ~ N(0,1)*0.05 for p<=q over the o+v
-    orbitals (occupied first), Q = 0..nQ-1;
+  * three-index factors B^Q_pq = B^Q_qp ~ N(0,1)*sigma for p<=q over the o+v
+    orbitals (occupied first), Q = 0..nQ-1; sigma defaults to
+    0.05*sqrt(150/(v*sqrt(nQ))), which keeps the effective coupling
+    v*sigma^2*sqrt(nQ) of the v=18/nQ=72 system (sigma ~ 0.05) at every size;
+    a fixed 0.05 makes CCSD diverge for v*sqrt(nQ) >> 150;
   * Mulliken (pq|rs) = sum_Q B^Q_pq B^Q_rs (8-fold symmetric, PSD).
 
 The random stream is counter based (splitmix64 finaliser + Box-Muller), so the
@@ -63,16 +66,21 @@ def orbital_energies(seed, o, v):
     return eo, ev
 
 
-def three_index(seed, o, v, nQ):
+def default_sigma(v, nQ):
+    return 0.05 * np.sqrt(150.0 / (v * np.sqrt(nQ)))
+
+
+def three_index(seed, o, v, nQ, sigma=0.0):
     """B[Q,p,q] (symmetric in p,q).  Counter of (Q, p<=q) is
     Q*npair + p*n - p*(p-1)/2 + (q-p)."""
+    sg = sigma if sigma > 0 else default_sigma(v, nQ)
     n = o + v
     npair = n * (n + 1) // 2
     p, q = np.triu_indices(n)
     t = p * n - p * (p - 1) // 2 + (q - p)
     B = np.empty((nQ, n, n))
     for Q in range(nQ):
-        z = 0.05 * normals(seed, STREAM_B, np.uint64(Q * npair) + t.astype(np.uint64))
+        z = sg * normals(seed, STREAM_B, np.uint64(Q * npair) + t.astype(np.uint64))
         B[Q, p, q] = z
         B[Q, q, p] = z
     return B
@@ -117,9 +125,9 @@ def blocks_from_eri(eri, o):
     }
 
 
-def synthetic_system(seed, o, v, nQ=None):
+def synthetic_system(seed, o, v, nQ=None, sigma=0.0):
     """(f, V, B, eps) of the synthetic construction; nQ defaults to 3(o+v)."""
     nQ = 3 * (o + v) if nQ is None else nQ
     eo, ev = orbital_energies(seed, o, v)
-    B = three_index(seed, o, v, nQ)
+    B = three_index(seed, o, v, nQ, sigma)
     return fock(eo, ev), blocks_from_B(B, o), B, (eo, ev)
