@@ -1,0 +1,212 @@
+"""Benchmark: closed-shell CCSD iterations/s on one MI355X, mixed precision vs FP64.
+
+Workload: the seeded synthetic closed-shell system o=48, v=288, nQ=1008
+(eps_occ ~ U[-1.5,-0.3], eps_vir ~ U[0.2,5.0], (pq|rs) = sum_Q B^Q_pq B^Q_rs
+with the size-normalised factor scale), integrals and amplitudes resident in
+HBM (the full 55 GB FP64 vvvv block included).
+
+A "step" is one CCSD iteration (RHFCCSD::iterate: intermediates, residual,
+update, energy, DIIS).  The FP64 solve runs to convergence first (the
+reference arithmetic); the mixed solve (BF16x3 MFMA contractions with FP32
+accumulation until the residual drops below 1e-5, FP64 afterwards) then runs
+W untimed warmup iterations and exactly K timed iterations, and continues
+untimed to convergence so its energy can be compared with the FP64 one.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this engine runs on one
+GPU (see DESIGN.md for how it would shard).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "CCSD iter/s mixed vs FP64 at |ΔE|≤1e-6 Eh; served jobs/h + p99 time-to-E"
+PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
+
+
+def flops_per_iter(o, v):
+    """Algorithmic FLOPs of one iteration, term by term from rhfccsd.cxx:231-326."""
+    return (2 * o**2 * v**4 + 12 * o**3 * v**3 + 14 * o**2 * v**3 + 4 * o * v**3 + 4 * o**4 * v**2
+            + 20 * o**3 * v**2 + 6 * o**2 * v**2 + 2 * o**4 * v + 4 * o**3 * v + 4 * (o**2 * v + o * v**2))
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def roofline_of(timing, prefix, peak):
+    """Dominant GEMM class with the given prefix: algorithmic FLOPs per launch
+    over the mean launch time (HIP events on the context stream)."""
+    cands = {k: t for k, t in timing.items() if k.startswith(prefix) and t["launches"] > 0}
+    if not cands:
+        return None
+    name, t = max(cands.items(), key=lambda kv: kv[1]["total_ms"])
+    per_launch_flops = t["flops"] / t["launches"]
+    mean_s = t["total_ms"] / t["launches"] / 1e3
+    ach = per_launch_flops / mean_s
+    return {"kernel": name, "bound": "mfma", "achieved": ach / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": ach / peak, "traffic": None, "launches": t["launches"], "mean_ms": t["total_ms"] / t["launches"],
+            "flops_per_launch": per_launch_flops}
+
+
+def cpu_baseline(o_s, v_s, iters, seed, o, v):
+    """Oracle (numpy/OpenBLAS) iterations on a smaller system of the same
+    construction, scaled to the workload by the FLOP model."""
+    import numpy as np  # noqa: F401
+    from oracle import rhfccsd, synth
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max(int(i.get("num_threads", 1)) for i in threadpool_info()) or 1
+    except Exception:
+        threads = os.cpu_count() or 1
+    f, V, _, _ = synth.synthetic_system(seed, o_s, v_s)
+    cc = rhfccsd.RHFCCSD(f, V)
+    cc.iterate()   # warm (BLAS threads, page faults)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        cc.iterate()
+    dt = (time.perf_counter() - t0) / iters
+    scale = flops_per_iter(o_s, v_s) / flops_per_iter(o, v)
+    return {"value": scale / dt, "unit": "iter/s", "cores": threads, "kind": "port",
+            "sample": f"{iters} oracle iterations of the o={o_s}, v={v_s} synthetic system "
+                      f"({dt:.2f} s/iter, numpy+OpenBLAS on {threads} threads), scaled to o={o}, v={v} by the "
+                      f"FLOP ratio F({o_s},{v_s})/F({o},{v}) = {scale:.3e}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--o", type=int, default=48)
+    ap.add_argument("--v", type=int, default=288)
+    ap.add_argument("--nq", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--convergence", type=float, default=1e-9)
+    ap.add_argument("--diis", type=int, default=5)
+    ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--refine-below", type=float, default=1e-5)
+    ap.add_argument("--cpu-sample", default="16,112,2", help="o,v,iterations of the CPU oracle sample ('' to skip)")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solve")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"metric": METRIC, "error": "this engine runs on one GPU; multi-GPU sharding is a design "
+                          "(DESIGN.md), not a launchable path", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import aquarius_amd as aq
+
+    o, v = a.o, a.v
+    nq = a.nq or 3 * (o + v)
+    F = flops_per_iter(o, v)
+    ctx = aq.Context(0)
+    t0 = time.perf_counter()
+    ints = aq.Integrals.synthetic(ctx, a.seed, o, v, nq)
+    ctx.sync()
+    log(f"integrals o={o} v={v} nQ={nq} built in {time.perf_counter() - t0:.1f}s; "
+        f"HBM in use {ctx.memory()[0] / 2**30:.1f} GiB")
+
+    out = {}
+    # ---- FP64 reference solve (to convergence)
+    e64 = None
+    if not a.no_fp64:
+        cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=60)
+        ctx.enable_timing(True)
+        ctx.timing_reset()
+        t0 = time.perf_counter()
+        r64 = cc.run(callback=lambda h: log(f"fp64 it {h['iter']:2d} E={h['energy']:.12f} conv={h['conv']:.2e} "
+                                              f"{h['seconds'] * 1e3:.0f} ms"))
+        t64 = time.perf_counter() - t0
+        tim64 = ctx.timing()
+        e64 = r64["energy"]
+        its = [h["seconds"] for h in cc.history]
+        steady = its[1:] if len(its) > 2 else its
+        out["fp64"] = {"iterations": r64["iterations"], "converged": r64["converged"], "energy": e64,
+                       "time_to_E_s": t64, "iter_per_s": len(steady) / sum(steady),
+                       "ms_per_iter_median": 1e3 * float(np.median(steady)),
+                       "roofline": roofline_of(tim64, "gemm_f64", PEAK["fp64"])}
+        cc.close()
+
+    # ---- mixed solve: W warmup, exactly K timed iterations, then on to convergence
+    cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
+                    precision=a.mixed, refine_below=a.refine_below, min_reduced_flops=1e8)
+    ctx.enable_timing(True)
+    ctx.timing_reset()
+    hist = []
+    tstart = time.perf_counter()
+    for _ in range(a.warmup):
+        p = cc.current_precision()
+        e, c = cc.iterate()
+        hist.append((e, c, p))
+    ctx.timing_reset()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        p = cc.current_precision()
+        t1 = time.perf_counter()
+        e, c = cc.iterate()
+        hist.append((e, c, p))
+        log(f"mixed it {len(hist):2d} [{['fp64', 'fp32', 'bf16x3'][p]}] E={e:.12f} conv={c:.2e} "
+            f"{(time.perf_counter() - t1) * 1e3:.0f} ms")
+    ctx.sync()
+    t_timed = time.perf_counter() - t0
+    timm = ctx.timing()
+    n_conv = next((i + 1 for i, h in enumerate(hist) if h[1] < a.convergence), None)
+    while n_conv is None and len(hist) < 200:
+        e, c = cc.iterate()
+        hist.append((e, c, cc.current_precision()))
+        if c < a.convergence:
+            n_conv = len(hist)
+    t_mixed_to_E = time.perf_counter() - tstart
+    e_mixed = hist[(n_conv or len(hist)) - 1][0]
+    cc.close()
+
+    value = a.steps / t_timed
+    reduced = [h for h in hist[a.warmup:a.warmup + a.steps] if h[2] != aq.AQ_PREC_FP64]
+    rf_red = roofline_of(timm, "gemm_bf16x3" if a.mixed == "bf16x3" else "gemm_f32", PEAK[a.mixed])
+    rf_64 = roofline_of(timm, "gemm_f64", PEAK["fp64"])
+    # dominant kernel of the timed region
+    cands = [r for r in (rf_red, rf_64) if r]
+    dominant = max(cands, key=lambda r: r["launches"] * r["mean_ms"]) if cands else None
+
+    res = {
+        "metric": METRIC, "value": value, "unit": "iter/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * t_timed / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": f"{a.mixed}+fp64", "data": "synthetic",
+        "config": {"workload": f"synthetic closed-shell RHF-CCSD o={o} v={v} nQ={nq} seed={a.seed}, full vvvv "
+                               f"resident; {a.mixed} MFMA contractions until conv<{a.refine_below:g}, then FP64",
+                   "o": o, "v": v, "nQ": nq, "seed": a.seed, "sigma": aq.synth_default_sigma(v, nq),
+                   "convergence": a.convergence, "diis_order": a.diis, "flops_per_iter": F},
+        "roofline": dominant,
+        "mixed": {"iterations_to_converge": n_conv, "energy": e_mixed, "time_to_E_s": t_mixed_to_E,
+                  "reduced_iterations_in_timed_window": len(reduced), "roofline_reduced": rf_red,
+                  "roofline_fp64": rf_64},
+        "fp64": out.get("fp64"),
+    }
+    if e64 is not None:
+        res["dE_mixed_vs_fp64"] = e_mixed - e64
+        res["fp64_iter_per_s"] = out["fp64"]["iter_per_s"]
+        res["speedup_iter_per_s"] = value / out["fp64"]["iter_per_s"]
+        res["speedup_time_to_E"] = out["fp64"]["time_to_E_s"] / t_mixed_to_E
+    res["serving"] = None   # the continuous-batching serving loop is not built yet
+    ints.free()
+    ctx.close()
+
+    if a.cpu_sample:
+        so, sv, si = (int(x) for x in a.cpu_sample.split(","))
+        log(f"cpu baseline: {si} oracle iterations at o={so} v={sv}")
+        res["cpu_baseline"] = cpu_baseline(so, sv, si, a.seed, o, v)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()
