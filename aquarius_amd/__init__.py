@@ -32,7 +32,8 @@ class TimingRecord(C.Structure):
 class RHFCCSDConfig(C.Structure):
     _fields_ = [("convergence", C.c_double), ("max_iterations", C.c_int), ("diis_order", C.c_int),
                 ("diis_start", C.c_int), ("diis_damping", C.c_double), ("precision", C.c_int),
-                ("refine_below", C.c_double), ("min_reduced_flops", C.c_double), ("fused_update", C.c_int)]
+                ("refine_below", C.c_double), ("min_reduced_flops", C.c_double), ("fused_update", C.c_int),
+                ("diis_reset_on_refine", C.c_int)]
 
 
 ITER_CB = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)
@@ -74,6 +75,8 @@ _SIGS = {
     "aq_read_all": [_P, _P, _DP],
     "aq_write_all": [_P, _P, _DP],
     "aq_slice": [_P, C.c_double, _P, _I64P, C.c_double, _P, _I64P, _I64P],
+    "aq_div": [_P, C.c_double, _P, _P, C.c_double, _P],
+    "aq_invert": [_P, C.c_double, _P, C.c_double, _P],
     "aq_rhfccsd_default_config": [C.POINTER(RHFCCSDConfig)],
     "aq_rhfccsd_create": [_P, C.POINTER(_P), C.POINTER(_P), C.POINTER(RHFCCSDConfig), C.POINTER(_P)],
     "aq_rhfccsd_destroy": [_P],
@@ -199,6 +202,12 @@ class Context:
         r = C.c_double()
         _ck(lib().aq_norm(self.h, X.h, p, C.byref(r)))
         return r.value
+
+    def div(self, alpha, A, B, beta, Cc):
+        _ck(lib().aq_div(self.h, alpha, A.h, B.h, beta, Cc.h))
+
+    def invert(self, alpha, A, beta, Cc):
+        _ck(lib().aq_invert(self.h, alpha, A.h, beta, Cc.h))
 
     def slice(self, alpha, A, startA, beta, B, startB, extent):
         a = np.asarray(startA, dtype=np.int64)

@@ -214,6 +214,27 @@ void launch_fill(aq_ctx* ctx, double* x, int64_t n, double alpha, double beta) {
     AQ_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------- div / invert
+
+// C = beta*C + alpha*A/B (A null: alpha/B); untouched where |B| <= DBL_MIN
+__global__ __launch_bounds__(TPB) void k_div(const double* __restrict__ A, const double* __restrict__ B, double* C,
+                                             int64_t n, double alpha, double beta) {
+    for (int64_t i = int64_t(blockIdx.x) * TPB + threadIdx.x; i < n; i += int64_t(gridDim.x) * TPB) {
+        const double b = B[i];
+        if (fabs(b) > 2.2250738585072014e-308) {
+            const double q = A ? alpha * A[i] / b : alpha / b;
+            C[i] = beta * C[i] + q;
+        }
+    }
+}
+
+void launch_div(aq_ctx* ctx, const double* A, const double* B, double* C, int64_t n, double alpha, double beta) {
+    if (n <= 0) return;
+    TimedLaunch tl(ctx, "div", 0.0, 8.0 * n * (A ? 4 : 3));
+    hipLaunchKernelGGL(k_div, dim3(grid_for(n, TPB * 4)), dim3(TPB), 0, ctx->stream, A, B, C, n, alpha, beta);
+    AQ_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ weight
 
 struct WeightArgs {

@@ -180,6 +180,7 @@ void launch_strided_sum(aq_ctx* ctx, const StridedPlan& p, double alpha, const d
                         double beta, double* dst);
 void launch_strided_convert(aq_ctx* ctx, const StridedPlan& p, const double* src, void* dst, Fmt fmt,
                             int64_t plane_stride);
+void launch_div(aq_ctx* ctx, const double* A, const double* B, double* C, int64_t n, double alpha, double beta);
 void launch_weight(aq_ctx* ctx, double* x, int ndim, const int64_t* len, const double* d_dev,
                    const int64_t* d_off, double shift);
 // deterministic reductions into ctx->h_result[slot]; caller syncs

@@ -782,6 +782,24 @@ int aq_write_all(aq_ctx* ctx, aq_tensor* X, const double* host) {
     });
 }
 
+int aq_div(aq_ctx* ctx, double alpha, const aq_tensor* A, const aq_tensor* B, double beta, aq_tensor* C) {
+    return guarded([&] {
+        AQ_CHECK(ctx && A && B && C, AQ_ERR_INVALID, "null argument");
+        AQ_CHECK(A->len == C->len && B->len == C->len, AQ_ERR_INVALID, "div needs tensors of equal shape");
+        launch_div(ctx, A->data, B->data, C->data, C->size, alpha, beta);
+        bump(C);
+    });
+}
+
+int aq_invert(aq_ctx* ctx, double alpha, const aq_tensor* A, double beta, aq_tensor* C) {
+    return guarded([&] {
+        AQ_CHECK(ctx && A && C, AQ_ERR_INVALID, "null argument");
+        AQ_CHECK(A->len == C->len, AQ_ERR_INVALID, "invert needs tensors of equal shape");
+        launch_div(ctx, nullptr, A->data, C->data, C->size, alpha, beta);
+        bump(C);
+    });
+}
+
 int aq_slice(aq_ctx* ctx, double alpha, const aq_tensor* A, const int64_t* startA, double beta, aq_tensor* B,
              const int64_t* startB, const int64_t* extent) {
     return guarded([&] {

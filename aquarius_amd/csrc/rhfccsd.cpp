@@ -207,6 +207,7 @@ struct aq_rhfccsd {
     std::unique_ptr<DIIS> diis;
     double mp2 = 0.0, energy = 0.0, conv = std::numeric_limits<double>::max();
     int iter = 0;
+    int last_prec = AQ_PREC_FP64;
 
     int next_precision() const {
         if (cfg.precision == AQ_PREC_FP64) return AQ_PREC_FP64;
@@ -276,6 +277,9 @@ struct aq_rhfccsd {
     // rhfccsd.cxx:194-327
     void iterate() {
         const int prec = next_precision();
+        if (cfg.diis_reset_on_refine && iter > 0 && prec == AQ_PREC_FP64 && last_prec != AQ_PREC_FP64)
+            diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));
+        last_prec = prec;
         tensor::check(aq_ctx_set_precision(ctx, prec, cfg.min_reduced_flops));
 
         Tau["abij"] = T2["abij"];                                  // :231
@@ -386,6 +390,7 @@ int aq_rhfccsd_default_config(aq_rhfccsd_config* c) {
         c->refine_below = 1e-5;
         c->min_reduced_flops = 1e8;
         c->fused_update = 1;
+        c->diis_reset_on_refine = 0;
     });
 }
 

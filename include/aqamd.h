@@ -153,6 +153,13 @@ int aq_write(aq_ctx* ctx, aq_tensor* X, int64_t n, const int64_t* keys, const do
 int aq_read_all(aq_ctx* ctx, const aq_tensor* X, double* host);
 int aq_write_all(aq_ctx* ctx, aq_tensor* X, const double* host);
 
+/* Elementwise C = beta*C + alpha*A/B over equal-shaped tensors; elements where
+ * |B| <= DBL_MIN are left unchanged.  CTFTensor::div (ctf_tensor.cxx:271-342). */
+int aq_div(aq_ctx* ctx, double alpha, const aq_tensor* A, const aq_tensor* B, double beta, aq_tensor* C);
+/* Elementwise C = beta*C + alpha/A; elements where |A| <= DBL_MIN are left
+ * unchanged.  CTFTensor::invert (ctf_tensor.cxx:344-371). */
+int aq_invert(aq_ctx* ctx, double alpha, const aq_tensor* A, double beta, aq_tensor* C);
+
 /* B[startB + r] = beta*B[startB + r] + alpha*A[startA + r] for r over the box
  * `extent` (CTFTensor::slice, ctf_tensor.cxx:219-256, NS only). */
 int aq_slice(aq_ctx* ctx, double alpha, const aq_tensor* A, const int64_t* startA,
@@ -178,6 +185,9 @@ typedef struct aq_rhfccsd_config {
     /* 1: fuse weight + update + Tau + energy + conv into one kernel;
      * 0: issue them as separate operator calls, as the reference does */
     int fused_update;
+    /* 1: drop the DIIS history when the contractions switch to FP64, so the
+     * refinement extrapolates only over FP64 residuals */
+    int diis_reset_on_refine;
 } aq_rhfccsd_config;
 
 int aq_rhfccsd_default_config(aq_rhfccsd_config* cfg);

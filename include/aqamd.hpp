@@ -143,10 +143,10 @@ class Tensor {
         check(aq_tensor_lengths(h_, l.data()));
         return l;
     }
-    // implicit index string "abc..." (indexable_tensor.hpp:58)
+    // implicit index string "ABC..." (indexable_tensor.hpp:58-63)
     std::string implicit() const {
         std::string s;
-        for (int k = 0; k < ndim(); ++k) s.push_back(char('a' + k));
+        for (int k = 0; k < ndim(); ++k) s.push_back(char('A' + k));
         return s;
     }
 

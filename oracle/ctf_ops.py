@@ -19,7 +19,7 @@ import numpy as np
 from numpy.lib.stride_tricks import as_strided
 
 __all__ = ["label_view", "contract", "tsum", "scale", "fill", "dot", "weight",
-           "norm", "to_flat", "from_flat", "read_keys", "write_keys"]
+           "norm", "div", "invert", "to_flat", "from_flat", "read_keys", "write_keys"]
 
 
 def label_view(arr, labels):
@@ -155,6 +155,23 @@ def norm(X, p):
     if p == 2:
         return float(np.sqrt(np.sum(X * X)))
     raise ValueError("p must be 0, 1 or 2")
+
+
+DBL_MIN = np.finfo(np.float64).tiny
+
+
+def div(alpha, A, B, beta, C):
+    """C = beta*C + alpha*A/B elementwise, skipping |B| <= DBL_MIN
+    (CTFTensor::div, ctf_tensor.cxx:271-342)."""
+    m = np.abs(B) > DBL_MIN
+    C[m] = beta * C[m] + alpha * A[m] / B[m]
+
+
+def invert(alpha, A, beta, C):
+    """C = beta*C + alpha/A elementwise, skipping |A| <= DBL_MIN
+    (CTFTensor::invert, ctf_tensor.cxx:344-371)."""
+    m = np.abs(A) > DBL_MIN
+    C[m] = beta * C[m] + alpha / A[m]
 
 
 def to_flat(X):

@@ -228,6 +228,25 @@ def test_dot_norm_fill_io_slice(ctx):
         t.free()
 
 
+def test_div_invert_bit_exact(ctx):
+    rng = np.random.default_rng(12)
+    A = rnd(rng, (7, 5, 3))
+    B = rnd(rng, (7, 5, 3))
+    B[0, 0, 0] = 0.0
+    B[1, 2, 0] = 1e-310
+    Cm = rnd(rng, (7, 5, 3))
+    ref = Cm.copy()
+    ops.div(1.5, A, B, 0.25, ref)
+    tA, tB, tC = ctx.from_numpy(A), ctx.from_numpy(B), ctx.from_numpy(Cm)
+    ctx.div(1.5, tA, tB, 0.25, tC)
+    assert np.array_equal(tC.numpy(), ref)
+    ops.invert(-2.0, B, 1.0, ref)
+    ctx.invert(-2.0, tB, 1.0, tC)
+    assert np.array_equal(tC.numpy(), ref)
+    for t in (tA, tB, tC):
+        t.free()
+
+
 def test_operand_cache_tracks_relabelling_and_writes(ctx):
     """A constant tensor reached with different label names, and then
     overwritten, must always be read through the right permutation/version."""

@@ -170,6 +170,17 @@ def test_norms_and_dot():
     assert ops.dot(X, "ab", Y, "ba") == 2.0 - 12.0 + 2.0 + 0.5
 
 
+def test_div_and_invert_skip_tiny_divisors():
+    A = np.array([1.0, 2.0, 3.0, 4.0])
+    B = np.array([2.0, 0.0, 1e-310, -4.0])
+    C = np.array([10.0, 20.0, 30.0, 40.0])
+    ops.div(3.0, A, B, 0.5, C)
+    np.testing.assert_array_equal(C, [5.0 + 1.5, 20.0, 30.0, 20.0 - 3.0])
+    D = np.array([1.0, 1.0, 1.0, 1.0])
+    ops.invert(2.0, B, 1.0, D)
+    np.testing.assert_array_equal(D, [2.0, 1.0, 1.0, 0.5])
+
+
 def test_keys_are_column_major():
     X = np.arange(24, dtype=float).reshape(2, 3, 4)
     # key = i0 + 2*i1 + 6*i2
