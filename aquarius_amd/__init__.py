@@ -51,6 +51,7 @@ _SIGS = {
     "aq_ctx_sync": [_P],
     "aq_ctx_set_precision": [_P, C.c_int, C.c_double],
     "aq_ctx_get_precision": [_P, C.POINTER(C.c_int)],
+    "aq_ctx_set_gemm_variant": [_P, C.c_int, C.c_int],
     "aq_ctx_memory": [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "aq_ctx_trim": [_P],
     "aq_ctx_enable_timing": [_P, C.c_int],
@@ -142,6 +143,9 @@ class Context:
     def set_precision(self, prec, min_reduced_flops=0.0):
         prec = PRECISIONS.get(prec, prec)
         _ck(lib().aq_ctx_set_precision(self.h, int(prec), float(min_reduced_flops)))
+
+    def set_gemm_variant(self, prec, variant):
+        _ck(lib().aq_ctx_set_gemm_variant(self.h, int(PRECISIONS.get(prec, prec)), int(variant)))
 
     def memory(self):
         a, b = C.c_size_t(), C.c_size_t()

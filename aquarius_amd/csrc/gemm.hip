@@ -76,13 +76,83 @@ template <> struct Mfma<float> {
     static __device__ __forceinline__ int drow(int lane, int r) { return 4 * (lane / 16) + r; }
 };
 
-template <typename T, int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-    constexpr int BK = 16;
+// Element e (= tid + 256 r) of a ROWS x BK operand tile: its row and k index.
+// KFAST: the operand is stored with k contiguous (rows are strided).
+template <int ROWS, int BK, bool KFAST>
+__device__ __forceinline__ void tile_coord(int e, int& row, int& kk) {
+    if (KFAST) { kk = e % BK; row = e / BK; } else { row = e % ROWS; kk = e / ROWS; }
+}
+
+// Register-staged loader of one ROWS x BK operand tile (256 threads, every
+// thread ROWS*BK/256 elements).  KC: the operand is stored k-contiguous
+// (element (row,k) at row*ld + k), otherwise row-contiguous (k*ld + row).
+// Each thread keeps one base pointer, a fixed coordinate and a validity mask;
+// out-of-range elements read an in-bounds clamped address and are zeroed, so
+// the loads carry no branches.
+template <typename T, int ROWS, int BK, bool KC>
+struct TileLoader {
+    static_assert((ROWS * BK) % 256 == 0, "tile must divide over 256 threads");
+    static constexpr int NE = ROWS * BK / 256;
+    static constexpr int VSTEP = 256 / (KC ? BK : ROWS);   // step of the varying coordinate per element
+    const T* base;      // element (row0 + fixed/var0 ..., kbeg + ...) of element 0
+    const T* safe;      // any in-bounds element
+    int64_t ld;
+    int fixed, var0;
+    unsigned rowmask;   // KC: bit r = row of element r in range; !KC: bit 0 = the fixed row in range
+    T v[NE];
+
+    __device__ __forceinline__ void init(const T* P, int64_t ld_, int64_t row0, int64_t nrows, int64_t kbeg, int tid) {
+        ld = ld_;
+        safe = P;
+        if (KC) {
+            fixed = tid % BK;        // k within the tile
+            var0 = tid / BK;         // row of element 0
+            base = P + (row0 + var0) * ld + kbeg + fixed;
+            rowmask = 0;
+#pragma unroll
+            for (int r = 0; r < NE; ++r)
+                if (row0 + var0 + r * VSTEP < nrows) rowmask |= 1u << r;
+        } else {
+            fixed = tid % ROWS;      // row within the tile
+            var0 = tid / ROWS;       // k of element 0
+            base = P + (kbeg + var0) * ld + row0 + fixed;
+            rowmask = (row0 + fixed < nrows) ? 1u : 0u;
+        }
+    }
+    // load k-tile kt (k0 = kbeg + kt*BK) into registers
+    __device__ __forceinline__ void load(int64_t kt, int64_t kbeg, int64_t kend) {
+        const int64_t k0 = kbeg + kt * BK;
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            bool ok;
+            const T* p;
+            if (KC) {
+                ok = ((rowmask >> r) & 1u) && (k0 + fixed < kend);
+                p = base + kt * BK + int64_t(r * VSTEP) * ld;
+            } else {
+                ok = rowmask && (k0 + var0 + r * VSTEP < kend);
+                p = base + (kt * BK + r * VSTEP) * ld;
+            }
+            const T x = *(ok ? p : safe);
+            v[r] = ok ? x : T(0);
+        }
+    }
+    // registers -> LDS tile S[k][row]
+    template <int LD>
+    __device__ __forceinline__ void store(T (*S)[LD]) const {
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            if (KC) S[fixed][var0 + r * VSTEP] = v[r];
+            else S[var0 + r * VSTEP][fixed] = v[r];
+        }
+    }
+};
+
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int BK, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
     constexpr int PAD = 1;
     constexpr int WTM = BM / WM, WTN = BN / WN;   // wave tile
     constexpr int FM = WTM / 16, FN = WTN / 16;   // 16x16 blocks per wave
-    constexpr int NA = (BM * BK + 255) / 256, NB = (BN * BK + 255) / 256;
     static_assert(WM * WN == 4, "4 waves");
     typedef typename Mfma<T>::acc_t acc_t;
 
@@ -102,51 +172,18 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
     const int64_t kbeg = int64_t(split) * g.kchunk;
     const int64_t kend = min(g.K, kbeg + g.kchunk);
 
-    const T* A = static_cast<const T*>(g.A) + int64_t(b) * g.strideA;
-    const T* B = static_cast<const T*>(g.B) + int64_t(b) * g.strideB;
-
-    T ra[NA], rb[NB];
-    auto load = [&](int64_t k0) {
-#pragma unroll
-        for (int r = 0; r < NA; ++r) {
-            const int e = tid + 256 * r;
-            int mm, kk;
-            if (g.transA) { kk = e % BK; mm = e / BK; } else { mm = e % BM; kk = e / BM; }
-            const int64_t m = m0 + mm, k = k0 + kk;
-            T v = T(0);
-            if (e < BM * BK && m < g.M && k < kend)
-                v = g.transA ? A[k + m * g.lda] : A[m + k * g.lda];
-            ra[r] = v;
-        }
-#pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            const int e = tid + 256 * r;
-            int nn, kk;
-            if (g.transB) { nn = e % BN; kk = e / BN; } else { kk = e % BK; nn = e / BK; }
-            const int64_t n = n0 + nn, k = k0 + kk;
-            T v = T(0);
-            if (e < BN * BK && n < g.N && k < kend)
-                v = g.transB ? B[n + k * g.ldb] : B[k + n * g.ldb];
-            rb[r] = v;
-        }
+    // opA(m,k) = A[k + m*lda] when TA (k contiguous); opB(k,n) = B[k + n*ldb] when !TB (k contiguous)
+    TileLoader<T, BM, BK, TA> la;
+    TileLoader<T, BN, BK, !TB> lb;
+    la.init(static_cast<const T*>(g.A) + int64_t(b) * g.strideA, g.lda, m0, g.M, kbeg, tid);
+    lb.init(static_cast<const T*>(g.B) + int64_t(b) * g.strideB, g.ldb, n0, g.N, kbeg, tid);
+    auto load = [&](int64_t kt) {
+        la.load(kt, kbeg, kend);
+        lb.load(kt, kbeg, kend);
     };
     auto store = [&](int buf) {
-#pragma unroll
-        for (int r = 0; r < NA; ++r) {
-            const int e = tid + 256 * r;
-            if (e >= BM * BK) continue;
-            int mm, kk;
-            if (g.transA) { kk = e % BK; mm = e / BK; } else { mm = e % BM; kk = e / BM; }
-            As[buf][kk][mm] = ra[r];
-        }
-#pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            const int e = tid + 256 * r;
-            if (e >= BN * BK) continue;
-            int nn, kk;
-            if (g.transB) { nn = e % BN; kk = e / BN; } else { kk = e % BK; nn = e / BK; }
-            Bs[buf][kk][nn] = rb[r];
-        }
+        la.store(As[buf]);
+        lb.store(Bs[buf]);
     };
 
     acc_t acc[FM][FN];
@@ -156,12 +193,15 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
     const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    if (nk > 0) load(kbeg);
+    if (nk > 0) load(0);
     for (int64_t kt = 0; kt < nk; ++kt) {
         const int buf = int(kt & 1);
         store(buf);
         __syncthreads();
-        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
+        // next tile's global loads go out before this tile's MFMAs, so their
+        // latency hides behind the matrix work
+        if (kt + 1 < nk) load(kt + 1);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + lane / 16;
@@ -350,8 +390,33 @@ struct TileCfg {
 };
 
 template <typename T, int BM, int BN, int WM, int WN>
-static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
-    hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN>), grid, dim3(256), 0, ctx->stream, a);
+static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid, int variant) {
+#define AQ_LAUNCH(BK_, MW_)                                                                                       \
+    do {                                                                                                          \
+        if (a.transA && a.transB)                                                                                 \
+            hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN, true, true, BK_, MW_>), grid, dim3(256), 0, ctx->stream, a);   \
+        else if (a.transA)                                                                                        \
+            hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN, true, false, BK_, MW_>), grid, dim3(256), 0, ctx->stream, a);  \
+        else if (a.transB)                                                                                        \
+            hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN, false, true, BK_, MW_>), grid, dim3(256), 0, ctx->stream, a);  \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN, false, false, BK_, MW_>), grid, dim3(256), 0, ctx->stream, a); \
+    } while (0)
+    // variants of the large tile: 0 = BK16 one wave/SIMD, 1 = BK32, 2 = BK16 register-lean (two waves/SIMD)
+    if (BM == 128 && BN == 128 && variant == 1) AQ_LAUNCH(32, 1);
+    else if (BM == 128 && BN == 128 && variant == 2) AQ_LAUNCH(16, 2);
+    else AQ_LAUNCH(16, 1);
+#undef AQ_LAUNCH
+}
+
+// Measured on MI355X (square 8192^3 / ladder / ring shapes): the register-lean
+// BK-16 variant with two waves per SIMD hides the global-load latency that a
+// single wave per SIMD exposes (FP64 35 -> 56 TF/s; FP32 unchanged at ~108).
+int default_gemm_variant(Fmt fmt) { return fmt == FMT_BF16X2 ? 0 : 2; }
+
+int gemm_kchunk_bk(Fmt fmt, int BM, int BN, int variant) {
+    if (fmt == FMT_BF16X2) return 32;
+    return (BM == 128 && BN == 128 && variant == 1) ? 32 : 16;
 }
 
 template <int BM, int BN, int WM, int WN>
@@ -375,8 +440,9 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     const int64_t tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
     AQ_CHECK(tiles < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "GEMM grid too large");
 
+    const int variant = ctx->gemm_variant[d.fmt] >= 0 ? ctx->gemm_variant[d.fmt] : default_gemm_variant(d.fmt);
     // split-K when the output alone cannot fill the device
-    const int BK = d.fmt == FMT_BF16X2 ? 32 : 16;
+    const int BK = gemm_kchunk_bk(d.fmt, BM, BN, variant);
     int nsplit = 1;
     const int64_t ktiles = (d.K + BK - 1) / BK;
     const int64_t want = 1024;
@@ -414,15 +480,15 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     {
         TimedLaunch tl(ctx, name, flops, bytes);
         if (d.fmt == FMT_F64) {
-            if (BM == 128 && BN == 128) run_cfg<double, 128, 128, 2, 2>(ctx, a, grid);
-            else if (BM == 64) run_cfg<double, 64, 64, 2, 2>(ctx, a, grid);
-            else if (BN == 16) run_cfg<double, 128, 16, 4, 1>(ctx, a, grid);
-            else run_cfg<double, 16, 128, 1, 4>(ctx, a, grid);
+            if (BM == 128 && BN == 128) run_cfg<double, 128, 128, 2, 2>(ctx, a, grid, variant);
+            else if (BM == 64) run_cfg<double, 64, 64, 2, 2>(ctx, a, grid, 0);
+            else if (BN == 16) run_cfg<double, 128, 16, 4, 1>(ctx, a, grid, 0);
+            else run_cfg<double, 16, 128, 1, 4>(ctx, a, grid, 0);
         } else if (d.fmt == FMT_F32) {
-            if (BM == 128 && BN == 128) run_cfg<float, 128, 128, 2, 2>(ctx, a, grid);
-            else if (BM == 64) run_cfg<float, 64, 64, 2, 2>(ctx, a, grid);
-            else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid);
-            else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid);
+            if (BM == 128 && BN == 128) run_cfg<float, 128, 128, 2, 2>(ctx, a, grid, variant);
+            else if (BM == 64) run_cfg<float, 64, 64, 2, 2>(ctx, a, grid, 0);
+            else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
+            else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
         } else {
             if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
             else if (BM == 64) run_cfg_bf16<64, 64, 2, 2>(ctx, a, grid);

@@ -104,6 +104,7 @@ struct aq_ctx {
     size_t tensor_bytes = 0;
     int precision = AQ_PREC_FP64;
     double min_reduced_flops = 0.0;
+    int gemm_variant[3] = {-1, -1, -1};   // per format; -1 = library default
     uint64_t next_id = 1;
 
     // scratch pool: exact-size free lists

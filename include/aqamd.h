@@ -75,6 +75,11 @@ int aq_ctx_sync(aq_ctx* ctx);
  * reduced precision is selected. */
 int aq_ctx_set_precision(aq_ctx* ctx, int precision, double min_reduced_flops);
 int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
+/* Kernel variant of the large-tile GEMM for one arithmetic (AQ_PREC_*):
+ * -1 = library default, 0 = BK 16, 1 = BK 32, 2 = BK 16 with two waves per
+ * SIMD (BF16x3 has variant 0 only).  A tuning knob: results are identical up
+ * to floating-point summation order. */
+int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
 /* Bytes currently held (tensors + caches + pooled scratch) and the budget. */
 int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget);
 /* Drop cached operand copies and pooled scratch (keeps tensors). */
