@@ -33,7 +33,7 @@ class RHFCCSDConfig(C.Structure):
     _fields_ = [("convergence", C.c_double), ("max_iterations", C.c_int), ("diis_order", C.c_int),
                 ("diis_start", C.c_int), ("diis_damping", C.c_double), ("precision", C.c_int),
                 ("refine_below", C.c_double), ("min_reduced_flops", C.c_double), ("fused_update", C.c_int),
-                ("diis_reset_on_refine", C.c_int)]
+                ("diis_reset_on_refine", C.c_int), ("fp32_stage_below", C.c_double)]
 
 
 ITER_CB = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)
@@ -390,3 +390,92 @@ class RHFCCSD:
         r = C.c_int()
         _ck(lib().aq_rhfccsd_current_precision(self.h, C.byref(r)))
         return r.value
+
+
+# ---------------------------------------------------------------- serving
+
+class JobSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("o", C.c_int), ("v", C.c_int), ("nQ", C.c_int), ("sigma", C.c_double),
+                ("cfg", RHFCCSDConfig)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("slot", C.c_int), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_end", C.c_double), ("energy", C.c_double), ("conv", C.c_double),
+                ("mp2", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
+                ("reduced_iterations", C.c_int), ("bytes_reserved", C.c_size_t), ("bytes_peak", C.c_size_t),
+                ("error_code", C.c_int), ("error", C.c_char * 160)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "error"}
+        d["error"] = self.error.decode(errors="replace")
+        return d
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("queued", C.c_int64),
+                ("running", C.c_int64), ("budget", C.c_size_t), ("reserved", C.c_size_t), ("elapsed_s", C.c_double)]
+
+
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, 3
+
+_SIGS.update({
+    "aq_ctx_peak_memory": [_P, C.POINTER(C.c_size_t), C.c_int],
+    "aq_job_bytes_estimate": [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int],
+    "aq_server_create": [C.c_int, C.c_size_t, C.c_int, C.c_double, C.POINTER(_P)],
+    "aq_server_submit": [_P, C.POINTER(JobSpec), C.POINTER(C.c_int64)],
+    "aq_server_poll": [_P, C.c_int64, C.POINTER(JobResult)],
+    "aq_server_wait": [_P, C.c_int64, C.POINTER(JobResult)],
+    "aq_server_drain": [_P],
+    "aq_server_stats_get": [_P, C.POINTER(ServerStats)],
+    "aq_server_destroy": [_P],
+})
+_RESTYPE["aq_job_bytes_estimate"] = C.c_size_t
+
+
+def job_bytes_estimate(o, v, nQ=0, diis_order=8, precision="fp64"):
+    return lib().aq_job_bytes_estimate(o, v, nQ, diis_order, int(PRECISIONS.get(precision, precision)))
+
+
+class Server:
+    """aq_server: continuous batching of synthetic CCSD jobs on one device."""
+
+    def __init__(self, device=0, hbm_budget=0, slots=4, max_bypass_s=30.0):
+        self.h = _P()
+        _ck(lib().aq_server_create(device, int(hbm_budget), slots, float(max_bypass_s), C.byref(self.h)))
+
+    def submit(self, seed, o, v, nQ=0, sigma=0.0, config=None, **over):
+        spec = JobSpec(seed=seed, o=o, v=v, nQ=nQ, sigma=sigma,
+                       cfg=config if config is not None else default_config(**over))
+        jid = C.c_int64()
+        _ck(lib().aq_server_submit(self.h, C.byref(spec), C.byref(jid)))
+        return jid.value
+
+    def poll(self, jid):
+        r = JobResult()
+        _ck(lib().aq_server_poll(self.h, jid, C.byref(r)))
+        return r.as_dict()
+
+    def wait(self, jid):
+        r = JobResult()
+        _ck(lib().aq_server_wait(self.h, jid, C.byref(r)))
+        return r.as_dict()
+
+    def drain(self):
+        _ck(lib().aq_server_drain(self.h))
+
+    def stats(self):
+        s = ServerStats()
+        _ck(lib().aq_server_stats_get(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def close(self):
+        if self.h:
+            _ck(lib().aq_server_destroy(self.h))
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -15,10 +15,28 @@ void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 size_t ctx_in_use(const aq_ctx* ctx) { return ctx->tensor_bytes + ctx->pool_bytes + ctx->cache_bytes; }
 
+// Stream-ordered device memory: allocations and frees are ordered on the
+// context's stream, so one context releasing memory never synchronises the
+// device under other contexts (concurrent serving).
+void* dmalloc(aq_ctx* ctx, size_t bytes) {
+    void* p = nullptr;
+    if (hipMallocAsync(&p, bytes, ctx->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void dfree(aq_ctx* ctx, void* p) {
+    if (p) (void)hipFreeAsync(p, ctx->stream);
+}
+
+static void note_peak(aq_ctx* ctx) { ctx->peak_bytes = std::max(ctx->peak_bytes, ctx_in_use(ctx)); }
+
 static void pool_release_all(aq_ctx* ctx) {
     for (auto& kv : ctx->pool_free) {
         for (void* p : kv.second) {
-            (void)hipFree(p);
+            dfree(ctx, p);
             ctx->pool_bytes -= kv.first;
         }
     }
@@ -27,17 +45,12 @@ static void pool_release_all(aq_ctx* ctx) {
 
 void evict_cache(aq_ctx* ctx, size_t need) {
     // least recently used first; copies pinned by the running operation stay
-    bool synced = false;
     auto it = ctx->cache_lru.end();
     while (it != ctx->cache_lru.begin() && ctx_in_use(ctx) + need > ctx->budget) {
         --it;
         auto ce = ctx->cache.find(*it);
         if (ce == ctx->cache.end() || ce->second.pins > 0) continue;
-        if (!synced) {
-            AQ_HIP(hipStreamSynchronize(ctx->stream));
-            synced = true;
-        }
-        (void)hipFree(ce->second.ptr);
+        dfree(ctx, ce->second.ptr);
         ctx->cache_bytes -= ce->second.bytes;
         ctx->cache.erase(ce);
         it = ctx->cache_lru.erase(it);
@@ -46,25 +59,19 @@ void evict_cache(aq_ctx* ctx, size_t need) {
 
 static void* raw_alloc(aq_ctx* ctx, size_t bytes) {
     if (ctx_in_use(ctx) + bytes > ctx->budget) {
-        AQ_HIP(hipStreamSynchronize(ctx->stream));
         pool_release_all(ctx);
         evict_cache(ctx, bytes);
     }
     AQ_CHECK(ctx_in_use(ctx) + bytes <= ctx->budget, AQ_ERR_OOM,
              "HBM budget exceeded: need " + std::to_string(bytes) + " B, in use " +
                  std::to_string(ctx_in_use(ctx)) + " of " + std::to_string(ctx->budget));
-    void* p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        (void)hipGetLastError();
-        AQ_HIP(hipStreamSynchronize(ctx->stream));
+    void* p = dmalloc(ctx, bytes);
+    if (!p) {
         pool_release_all(ctx);
         evict_cache(ctx, ctx->budget);   // drop every cached copy
-        e = hipMalloc(&p, bytes);
-        if (e != hipSuccess) {
-            (void)hipGetLastError();
-            throw Error(AQ_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " B failed");
-        }
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        p = dmalloc(ctx, bytes);
+        if (!p) throw Error(AQ_ERR_OOM, "device allocation of " + std::to_string(bytes) + " B failed");
     }
     return p;
 }
@@ -84,6 +91,7 @@ void* dev_alloc(aq_ctx* ctx, size_t bytes) {
     }
     void* p = raw_alloc(ctx, bytes);
     ctx->pool_bytes += bytes;
+    note_peak(ctx);
     return p;
 }
 
@@ -96,15 +104,17 @@ double* tensor_alloc(aq_ctx* ctx, size_t bytes) {
     bytes = std::max<size_t>(round_up(bytes), 256);
     void* p = raw_alloc(ctx, bytes);
     ctx->tensor_bytes += bytes;
+    note_peak(ctx);
     return static_cast<double*>(p);
 }
 
 void tensor_free(aq_ctx* ctx, double* p, size_t bytes) {
     bytes = std::max<size_t>(round_up(bytes), 256);
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipFree(p);
+    dfree(ctx, p);
     ctx->tensor_bytes -= bytes;
 }
+
+void note_cache_alloc(aq_ctx* ctx) { note_peak(ctx); }
 
 // ---------------------------------------------------------------- timing
 
@@ -166,6 +176,16 @@ int aq_ctx_create(int device, size_t hbm_budget, aq_ctx** out) {
         AQ_HIP(hipGetDeviceCount(&ndev));
         AQ_CHECK(device >= 0 && device < ndev, AQ_ERR_INVALID, "no such HIP device");
         AQ_HIP(hipSetDevice(device));
+        {
+            // keep memory released by hipFreeAsync in the device pool instead of
+            // returning it to the driver at every synchronisation
+            hipMemPool_t pool;
+            if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess) {
+                uint64_t thr = UINT64_MAX;
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+            }
+            (void)hipGetLastError();
+        }
         auto* ctx = new aq_ctx();
         ctx->device = device;
         try {
@@ -173,7 +193,8 @@ int aq_ctx_create(int device, size_t hbm_budget, aq_ctx** out) {
             size_t free_b = 0, total_b = 0;
             AQ_HIP(hipMemGetInfo(&free_b, &total_b));
             ctx->budget = hbm_budget ? hbm_budget : static_cast<size_t>(0.9 * double(free_b));
-            AQ_HIP(hipMalloc(&ctx->d_partials, sizeof(double) * (8 * 1024 + 64)));
+            ctx->d_partials = static_cast<double*>(dmalloc(ctx, sizeof(double) * (8 * 1024 + 64)));
+            AQ_CHECK(ctx->d_partials, AQ_ERR_OOM, "device allocation of reduction slots failed");
             AQ_HIP(hipHostMalloc(&ctx->h_result, sizeof(double) * 64, hipHostMallocDefault));
         } catch (...) {
             if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -191,9 +212,10 @@ int aq_ctx_destroy(aq_ctx* ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         resolve_timing(ctx);
         pool_release_all(ctx);
-        for (auto& kv : ctx->cache) (void)hipFree(kv.second.ptr);
+        for (auto& kv : ctx->cache) dfree(ctx, kv.second.ptr);
         for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
-        (void)hipFree(ctx->d_partials);
+        dfree(ctx, ctx->d_partials);
+        (void)hipStreamSynchronize(ctx->stream);
         (void)hipHostFree(ctx->h_result);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
@@ -244,12 +266,20 @@ int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget) {
     });
 }
 
+int aq_ctx_peak_memory(aq_ctx* ctx, size_t* peak, int reset) {
+    return guarded([&] {
+        AQ_CHECK(ctx && peak, AQ_ERR_INVALID, "null argument");
+        *peak = ctx->peak_bytes;
+        if (reset) ctx->peak_bytes = ctx_in_use(ctx);
+    });
+}
+
 int aq_ctx_trim(aq_ctx* ctx) {
     return guarded([&] {
         AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
         AQ_HIP(hipStreamSynchronize(ctx->stream));
         pool_release_all(ctx);
-        for (auto& kv : ctx->cache) (void)hipFree(kv.second.ptr);
+        for (auto& kv : ctx->cache) dfree(ctx, kv.second.ptr);
         ctx->cache.clear();
         ctx->cache_lru.clear();
         ctx->cache_bytes = 0;

@@ -476,7 +476,10 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     const double bytes = double(fmt_bytes(d.fmt)) * (double(d.M) * d.K + double(d.N) * d.K) * d.batch +
                          8.0 * double(d.M) * d.N * d.batch * (d.beta != 0.0 ? 2 : 1);
     char name[64];
-    snprintf(name, sizeof(name), "%s_%dx%d", d.tag ? d.tag : "gemm", BM, BN);
+    // one timing class per kernel instantiation: tile and operand layouts
+    // (N = opA rows / opB cols contiguous, T = k contiguous)
+    snprintf(name, sizeof(name), "%s_%dx%d_%c%c", d.tag ? d.tag : "gemm", BM, BN, d.transA ? 'T' : 'N',
+             d.transB ? 'T' : 'N');
     {
         TimedLaunch tl(ctx, name, flops, bytes);
         if (d.fmt == FMT_F64) {

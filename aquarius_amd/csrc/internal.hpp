@@ -102,6 +102,7 @@ struct aq_ctx {
     hipStream_t stream = nullptr;
     size_t budget = 0;
     size_t tensor_bytes = 0;
+    size_t peak_bytes = 0;           // high-water mark of tensors + pool + caches
     int precision = AQ_PREC_FP64;
     double min_reduced_flops = 0.0;
     int gemm_variant[3] = {-1, -1, -1};   // per format; -1 = library default
@@ -130,6 +131,9 @@ struct aq_ctx {
 namespace aqamd {
 
 // ---- memory (context.cpp)
+void* dmalloc(aq_ctx* ctx, size_t bytes);          // stream-ordered raw allocation (null on failure)
+void dfree(aq_ctx* ctx, void* p);                    // stream-ordered free
+void note_cache_alloc(aq_ctx* ctx);                  // peak tracking after a cache allocation
 void* dev_alloc(aq_ctx* ctx, size_t bytes);        // pooled scratch
 void dev_free(aq_ctx* ctx, void* p, size_t bytes);   // back to the pool
 double* tensor_alloc(aq_ctx* ctx, size_t bytes);     // persistent (counted against budget)

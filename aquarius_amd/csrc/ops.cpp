@@ -251,15 +251,14 @@ static const void* materialize(aq_ctx* ctx, const Labeled& src, const std::strin
         }
         if (it != ctx->cache.end()) {   // size changed (cannot happen for a live tensor id)
             ctx->cache_lru.erase(it->second.lru);
-            AQ_HIP(hipStreamSynchronize(ctx->stream));
-            (void)hipFree(it->second.ptr);
+            dfree(ctx, it->second.ptr);
             ctx->cache_bytes -= it->second.bytes;
             ctx->cache.erase(it);
         }
         if (ctx_in_use(ctx) + bytes > ctx->budget) evict_cache(ctx, bytes);
         if (ctx_in_use(ctx) + bytes <= ctx->budget) {
-            void* ptr = nullptr;
-            if (hipMalloc(&ptr, bytes) == hipSuccess) {
+            void* ptr = dmalloc(ctx, bytes);
+            if (ptr) {
                 CacheEntry e;
                 e.ptr = ptr;
                 e.bytes = bytes;
@@ -267,6 +266,7 @@ static const void* materialize(aq_ctx* ctx, const Labeled& src, const std::strin
                 e.lru = ctx->cache_lru.begin();
                 e.pins = 1;
                 ctx->cache_bytes += bytes;
+                note_cache_alloc(ctx);
                 clear(ptr);
                 launch_strided_convert(ctx, p, src.data, ptr, fmt, n);
                 e.version = src.t->version;
@@ -275,7 +275,6 @@ static const void* materialize(aq_ctx* ctx, const Labeled& src, const std::strin
                 pins.keys.push_back(key);
                 return ptr;
             }
-            (void)hipGetLastError();
         }
         // fall through: uncached scratch copy
     }
@@ -617,8 +616,7 @@ int aq_tensor_destroy(aq_tensor* t) {
         const std::string pre = std::to_string(t->id) + "|";
         for (auto it = ctx->cache.begin(); it != ctx->cache.end();) {
             if (it->first.compare(0, pre.size(), pre) == 0 && it->second.pins == 0) {
-                AQ_HIP(hipStreamSynchronize(ctx->stream));
-                (void)hipFree(it->second.ptr);
+                dfree(ctx, it->second.ptr);
                 ctx->cache_bytes -= it->second.bytes;
                 ctx->cache_lru.erase(it->second.lru);
                 it = ctx->cache.erase(it);

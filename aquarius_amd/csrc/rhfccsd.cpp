@@ -211,7 +211,12 @@ struct aq_rhfccsd {
 
     int next_precision() const {
         if (cfg.precision == AQ_PREC_FP64) return AQ_PREC_FP64;
-        if (cfg.refine_below > 0.0 && conv < cfg.refine_below) return AQ_PREC_FP64;
+        if (cfg.refine_below > 0.0 && conv < cfg.refine_below) {
+            // optional FP32 stage between the reduced phase and FP64 refinement
+            if (cfg.fp32_stage_below > 0.0 && conv >= cfg.fp32_stage_below && cfg.precision != AQ_PREC_FP32)
+                return AQ_PREC_FP32;
+            return AQ_PREC_FP64;
+        }
         return cfg.precision;
     }
 
@@ -391,6 +396,7 @@ int aq_rhfccsd_default_config(aq_rhfccsd_config* c) {
         c->min_reduced_flops = 1e8;
         c->fused_update = 1;
         c->diis_reset_on_refine = 0;
+        c->fp32_stage_below = 0.0;
     });
 }
 

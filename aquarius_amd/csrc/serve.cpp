@@ -1,0 +1,296 @@
+// Continuous-batching CCSD server (no reference counterpart: Aquarius runs one
+// task DAG at a time, src/task/task.cxx:495-588).
+//
+// `slots` worker threads, each owning one long-lived context (its own HIP
+// stream).  A job reserves its estimated device bytes against the server
+// budget when admitted and releases them when it finishes; admission scans
+// the queue in arrival order and starts the first job that fits (backfill),
+// except that once the oldest queued job has waited longer than
+// max_bypass_s no later job may overtake it.  Jobs run to convergence with
+// their integrals, amplitudes and DIIS history resident on the device; their
+// kernels overlap on the GPU through the per-slot streams.  Nothing mutable is
+// shared between jobs, so each job's arithmetic is independent of the mix.
+#include "internal.hpp"
+
+#include "../../include/aqamd.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+namespace aqamd {
+
+struct Job {
+    aq_job_spec spec{};
+    aq_job_result res{};
+    double t_queued = 0.0;
+};
+
+}  // namespace aqamd
+
+using namespace aqamd;
+
+struct aq_server {
+    int device = 0;
+    size_t budget = 0, reserved = 0;
+    double max_bypass_s = 0.0;
+    std::vector<aq_ctx*> ctxs;
+    std::vector<std::thread> workers;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;
+    std::map<int64_t, Job> jobs;
+    int64_t next_id = 1;
+    int64_t completed = 0, failed = 0, running = 0;
+    bool stopping = false;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+
+    // Pick the next admissible job (mu held); -1 if none.
+    int64_t pick() {
+        if (queue.empty()) return -1;
+        const int64_t head = queue.front();
+        const Job& hj = jobs[head];
+        const bool head_blocked = now() - hj.t_queued > max_bypass_s;
+        for (int64_t id : queue) {
+            const Job& j = jobs[id];
+            if (reserved + j.res.bytes_reserved <= budget) return id;
+            if (head_blocked) return -1;   // the oldest job must go first
+        }
+        return -1;
+    }
+
+    // Runs one job on the slot's context; fills `job.res` fields of a private
+    // copy (published under the lock by the caller).
+    void run_job(int slot, Job& job) {
+        aq_ctx* ctx = ctxs[slot];
+        const aq_job_spec& s = job.spec;
+        const int nQ = s.nQ > 0 ? s.nQ : 3 * (s.o + s.v);
+        ctx->budget = job.res.bytes_reserved;
+        ctx->peak_bytes = ctx_in_use(ctx);
+        std::vector<double> eps(size_t(s.o + s.v)), B(size_t(nQ) * (s.o + s.v) * (s.o + s.v));
+        tensor::check(aq_synth_host(s.seed, s.o, s.v, nQ, s.sigma, eps.data(), B.data()));
+        aq_tensor* ints[10] = {};
+        aq_rhfccsd* cc = nullptr;
+        auto cleanup = [&]() {
+            if (cc) aq_rhfccsd_destroy(cc);
+            for (auto* t : ints)
+                if (t) aq_tensor_destroy(t);
+            aq_ctx_trim(ctx);
+        };
+        try {
+            tensor::check(aq_synth_build(ctx, s.o, s.v, nQ, eps.data(), B.data(), ints));
+            std::vector<double>().swap(B);
+            tensor::check(aq_rhfccsd_create(ctx, ints, ints + 3, &s.cfg, &cc));
+            job.res.mp2 = 0.0;
+            tensor::check(aq_rhfccsd_mp2(cc, &job.res.mp2));
+            struct Count {
+                int reduced = 0;
+            } count;
+            auto cb = [](int, double, double, double, int precision, void* user) {
+                if (precision != AQ_PREC_FP64) static_cast<Count*>(user)->reduced += 1;
+            };
+            tensor::check(aq_rhfccsd_run(cc, cb, &count, &job.res.energy, &job.res.conv, &job.res.iterations,
+                                         &job.res.converged));
+            job.res.reduced_iterations = count.reduced;
+            job.res.bytes_peak = ctx->peak_bytes;
+        } catch (...) {
+            job.res.bytes_peak = ctx->peak_bytes;
+            cleanup();
+            throw;
+        }
+        cleanup();
+    }
+
+    void worker(int slot) {
+        (void)hipSetDevice(device);
+        for (;;) {
+            int64_t id = -1;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return stopping || (id = pick()) >= 0; });
+                if (id < 0 && stopping) return;
+                Job& j = jobs[id];
+                queue.erase(std::find(queue.begin(), queue.end(), id));
+                reserved += j.res.bytes_reserved;
+                j.res.status = AQ_JOB_RUNNING;
+                j.res.slot = slot;
+                j.res.t_start = now();
+                running += 1;
+            }
+            Job local;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                local = jobs[id];
+            }
+            int code = AQ_OK;
+            std::string msg;
+            try {
+                run_job(slot, local);
+            } catch (const Error& e) {
+                code = e.code;
+                msg = e.what();
+            } catch (const tensor::TensorError& e) {
+                code = e.code;
+                msg = e.what();
+            } catch (const std::exception& e) {
+                code = AQ_ERR_INVALID;
+                msg = e.what();
+            }
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                Job& j = jobs[id];
+                j.res = local.res;
+                j.res.t_end = now();
+                reserved -= j.res.bytes_reserved;
+                running -= 1;
+                if (code == AQ_OK) {
+                    j.res.status = AQ_JOB_DONE;
+                    completed += 1;
+                } else {
+                    j.res.status = AQ_JOB_FAILED;
+                    j.res.error_code = code;
+                    std::strncpy(j.res.error, msg.c_str(), sizeof(j.res.error) - 1);
+                    failed += 1;
+                }
+            }
+            cv_work.notify_all();
+            cv_done.notify_all();
+        }
+    }
+};
+
+extern "C" {
+
+size_t aq_job_bytes_estimate(int o, int v, int nQ, int diis_order, int precision) {
+    if (o <= 0 || v <= 0) return 0;
+    if (nQ <= 0) nQ = 3 * (o + v);
+    const double O = o, V = v, Q = nQ, k = std::max(diis_order, 1);
+    const double V4 = V * V * V * V, V3 = V * V * V * O, N2 = O * O * V * V, O3V = O * O * O * V, O4 = O * O * O * O;
+    const double ints = V4 + V3 + 3 * N2 + O3V + O4 + V * V + O * O + V * O;
+    const double solver = 9 * N2 + 2 * O3V + O4 + 4 * V * V;
+    const double diis = 2 * k * (N2 + V * O);
+    const double cache64 = 3 * V3 + 12 * N2;
+    const double cacheR = precision != AQ_PREC_FP64 ? 0.5 * (V4 + 3 * V3 + 12 * N2) : 0.0;
+    const double temps = 3 * N2;
+    const double solve = ints + solver + diis + cache64 + cacheR + temps;
+    const double build = ints + V4 + (V * V + V * O + O * O) * Q;
+    return size_t(8.0 * std::max(solve, build) * 1.10) + (size_t(64) << 20);
+}
+
+int aq_server_create(int device, size_t hbm_budget, int slots, double max_bypass_s, aq_server** out) {
+    return guarded([&] {
+        AQ_CHECK(out && slots >= 1 && slots <= 16, AQ_ERR_INVALID, "slots must be 1..16");
+        auto* s = new aq_server();
+        s->device = device;
+        s->max_bypass_s = max_bypass_s;
+        try {
+            for (int i = 0; i < slots; ++i) {
+                aq_ctx* c = nullptr;
+                tensor::check(aq_ctx_create(device, 0, &c));
+                s->ctxs.push_back(c);
+            }
+            size_t free_b = 0, total_b = 0;
+            AQ_HIP(hipMemGetInfo(&free_b, &total_b));
+            s->budget = hbm_budget ? std::min(hbm_budget, size_t(0.95 * double(free_b))) : size_t(0.9 * double(free_b));
+            for (int i = 0; i < slots; ++i) s->workers.emplace_back([s, i] { s->worker(i); });
+        } catch (...) {
+            for (auto* c : s->ctxs) aq_ctx_destroy(c);
+            delete s;
+            throw;
+        }
+        *out = s;
+    });
+}
+
+int aq_server_submit(aq_server* s, const aq_job_spec* spec, int64_t* job_id) {
+    return guarded([&] {
+        AQ_CHECK(s && spec && job_id, AQ_ERR_INVALID, "null argument");
+        AQ_CHECK(spec->o > 0 && spec->v > 0, AQ_ERR_INVALID, "job needs o > 0 and v > 0");
+        const size_t need = aq_job_bytes_estimate(spec->o, spec->v, spec->nQ, spec->cfg.diis_order,
+                                                  spec->cfg.precision);
+        std::lock_guard<std::mutex> lk(s->mu);
+        AQ_CHECK(!s->stopping, AQ_ERR_INVALID, "server is stopping");
+        AQ_CHECK(need <= s->budget, AQ_ERR_OOM,
+                 "job needs " + std::to_string(need) + " B, more than the server budget " + std::to_string(s->budget));
+        const int64_t id = s->next_id++;
+        Job& j = s->jobs[id];
+        j.spec = *spec;
+        j.res.id = id;
+        j.res.status = AQ_JOB_QUEUED;
+        j.res.slot = -1;
+        j.res.bytes_reserved = need;
+        j.res.t_submit = s->now();
+        j.t_queued = j.res.t_submit;
+        s->queue.push_back(id);
+        *job_id = id;
+        s->cv_work.notify_all();
+    });
+}
+
+int aq_server_poll(aq_server* s, int64_t id, aq_job_result* out) {
+    return guarded([&] {
+        AQ_CHECK(s && out, AQ_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(id);
+        AQ_CHECK(it != s->jobs.end(), AQ_ERR_INVALID, "unknown job id");
+        *out = it->second.res;
+    });
+}
+
+int aq_server_wait(aq_server* s, int64_t id, aq_job_result* out) {
+    return guarded([&] {
+        AQ_CHECK(s, AQ_ERR_INVALID, "null server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->jobs.find(id);
+        AQ_CHECK(it != s->jobs.end(), AQ_ERR_INVALID, "unknown job id");
+        s->cv_done.wait(lk, [&] { return it->second.res.status >= AQ_JOB_DONE; });
+        if (out) *out = it->second.res;
+    });
+}
+
+int aq_server_drain(aq_server* s) {
+    return guarded([&] {
+        AQ_CHECK(s, AQ_ERR_INVALID, "null server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.wait(lk, [&] { return s->queue.empty() && s->running == 0; });
+    });
+}
+
+int aq_server_stats_get(aq_server* s, aq_server_stats* out) {
+    return guarded([&] {
+        AQ_CHECK(s && out, AQ_ERR_INVALID, "null argument");
+        std::lock_guard<std::mutex> lk(s->mu);
+        out->submitted = s->next_id - 1;
+        out->completed = s->completed;
+        out->failed = s->failed;
+        out->queued = int64_t(s->queue.size());
+        out->running = s->running;
+        out->budget = s->budget;
+        out->reserved = s->reserved;
+        out->elapsed_s = s->now();
+    });
+}
+
+int aq_server_destroy(aq_server* s) {
+    return guarded([&] {
+        if (!s) return;
+        aq_server_drain(s);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stopping = true;
+        }
+        s->cv_work.notify_all();
+        for (auto& t : s->workers) t.join();
+        for (auto* c : s->ctxs) aq_ctx_destroy(c);
+        delete s;
+    });
+}
+
+}  // extern "C"

@@ -81,15 +81,86 @@ def cpu_baseline(o_s, v_s, iters, seed, o, v):
                       f"FLOP ratio F({o_s},{v_s})/F({o},{v}) = {scale:.3e}"}
 
 
+def job_stream(seed, n, o_min, o_max, rate):
+    """The serving workload: o ~ U{o_min..o_max}, v = round(o * U[4,8]),
+    Poisson arrivals at `rate` jobs/s, one synthetic system per job."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    o = rng.integers(o_min, o_max + 1, size=n)
+    v = np.rint(o * rng.uniform(4.0, 8.0, size=n)).astype(int)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    t -= t[0]
+    return [(int(1000 + i), int(o[i]), int(v[i]), float(t[i])) for i in range(n)]
+
+
+def serving(a):
+    """Continuous-batching serving run; prints one JSON line."""
+    import numpy as np
+    import aquarius_amd as aq
+    jobs = job_stream(a.seed, a.jobs, a.o_min, a.o_max, a.rate)
+    cfg = aq.default_config(convergence=a.convergence, diis_order=a.diis, max_iterations=150, precision=a.mixed,
+                            refine_below=a.refine_below, min_reduced_flops=1e8)
+    res = []
+    with aq.Server(slots=a.slots, hbm_budget=int(a.budget_gb * 2**30), max_bypass_s=a.max_bypass) as srv:
+        t0 = time.perf_counter()
+        ids = []
+        for seed, o, v, t in jobs:
+            dt = t - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(seed, o, v, config=cfg))
+        for k, i in enumerate(ids):
+            r = srv.wait(i)
+            res.append(r)
+            if k % 16 == 15 or k == len(ids) - 1:
+                log(f"served {k + 1}/{len(ids)} (last: o={jobs[k][1]} v={jobs[k][2]} E={r['energy']:.8f} "
+                    f"time-to-E {r['t_end'] - r['t_submit']:.2f}s)")
+        wall = max(r["t_end"] for r in res) - min(r["t_submit"] for r in res)
+    ok = [r for r in res if r["status"] == aq.JOB_DONE and r["converged"]]
+    tte = np.array([r["t_end"] - r["t_submit"] for r in ok])
+    out = {"metric": "served jobs/h + p99 time-to-E", "value": 3600.0 * len(ok) / wall, "unit": "jobs/h",
+           "n_gpus": 1, "higher_is_better": True, "dtype": f"{a.mixed}+fp64", "data": "synthetic",
+           "config": {"workload": f"{a.jobs} synthetic CCSD jobs, o~U{{{a.o_min}..{a.o_max}}}, v=round(o*U[4,8]), "
+                                  f"Poisson arrivals {a.rate}/s, HBM budget {a.budget_gb} GB, {a.slots} slots",
+                      "seed": a.seed, "convergence": a.convergence, "diis_order": a.diis},
+           "jobs": len(res), "converged": len(ok), "wall_s": wall,
+           "time_to_E_s": {"p50": float(np.percentile(tte, 50)), "p99": float(np.percentile(tte, 99)),
+                           "max": float(tte.max()), "mean": float(tte.mean())}}
+    # one-at-a-time FP64 solves of a sample of the same stream (no arrivals: back to back)
+    if a.fp64_sample:
+        sample = jobs[::max(1, len(jobs) // a.fp64_sample)][:a.fp64_sample]
+        c64 = aq.default_config(convergence=a.convergence, diis_order=a.diis, max_iterations=150)
+        times = []
+        with aq.Server(slots=1, hbm_budget=int(a.budget_gb * 2**30)) as srv:
+            for seed, o, v, _ in sample:
+                r = srv.wait(srv.submit(seed, o, v, config=c64))
+                times.append(r["t_end"] - r["t_start"])
+        mix_times = {jobs[k][0]: res[k]["t_end"] - res[k]["t_start"] for k in range(len(jobs))}
+        out["fp64_one_at_a_time"] = {"sample_jobs": len(sample), "mean_solve_s": float(np.mean(times)),
+                                     "jobs_per_h": 3600.0 / float(np.mean(times)),
+                                     "mixed_mean_solve_s_same_jobs": float(np.mean([mix_times[s[0]] for s in sample]))}
+        out["vs_fp64_one_at_a_time"] = out["value"] / out["fp64_one_at_a_time"]["jobs_per_h"]
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--serving", action="store_true", help="run the continuous-batching serving workload")
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--o-min", type=int, default=4)
+    ap.add_argument("--o-max", type=int, default=32)
+    ap.add_argument("--rate", type=float, default=4.0)
+    ap.add_argument("--budget-gb", type=float, default=240.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--max-bypass", type=float, default=30.0)
+    ap.add_argument("--fp64-sample", type=int, default=16)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0, help="timed iterations (0: the rest of the mixed solve)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--o", type=int, default=48)
     ap.add_argument("--v", type=int, default=288)
     ap.add_argument("--nq", type=int, default=0)
-    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--seed", type=int, default=None, help="default: 3 (single solve), 4 (serving stream)")
     ap.add_argument("--convergence", type=float, default=1e-9)
     ap.add_argument("--diis", type=int, default=8)
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"])
@@ -102,6 +173,12 @@ def main():
         print(json.dumps({"metric": METRIC, "error": "this engine runs on one GPU; multi-GPU sharding is a design "
                           "(DESIGN.md), not a launchable path", "n_gpus": a.gpus}))
         sys.exit(2)
+
+    if a.seed is None:
+        a.seed = 4 if a.serving else 3
+    if a.serving:
+        serving(a)
+        return
 
     import numpy as np
     import aquarius_amd as aq

@@ -82,6 +82,8 @@ int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
 int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
 /* Bytes currently held (tensors + caches + pooled scratch) and the budget. */
 int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget);
+/* High-water mark of the bytes held since creation or the last reset. */
+int aq_ctx_peak_memory(aq_ctx* ctx, size_t* peak, int reset);
 /* Drop cached operand copies and pooled scratch (keeps tensors). */
 int aq_ctx_trim(aq_ctx* ctx);
 /* Per-kernel-class timing with HIP events on the context's stream.
@@ -193,6 +195,10 @@ typedef struct aq_rhfccsd_config {
     /* 1: drop the DIIS history when the contractions switch to FP64, so the
      * refinement extrapolates only over FP64 residuals */
     int diis_reset_on_refine;
+    /* > 0: with a BF16x3 bulk phase, run FP32 contractions while
+     * fp32_stage_below <= conv < refine_below, and FP64 only below it
+     * (0 = switch straight to FP64 at refine_below) */
+    double fp32_stage_below;
 } aq_rhfccsd_config;
 
 int aq_rhfccsd_default_config(aq_rhfccsd_config* cfg);
@@ -241,6 +247,64 @@ double aq_synth_default_sigma(int v, int nQ);
  * out[3..9] = VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL. */
 int aq_synth_build(aq_ctx* ctx, int o, int v, int nQ, const double* eps, const double* B,
                    aq_tensor** out);
+
+/* ---- serving: continuous batching of CCSD jobs (no reference counterpart;
+ * Aquarius runs one task DAG sequentially, src/task/task.cxx:495-588) -------
+ *
+ * A server owns `slots` worker threads on one device, each with its own
+ * context (stream).  Jobs are admitted in arrival order against an HBM budget
+ * (the sum of the running jobs' reservations); a job that does not fit waits
+ * while later jobs that fit may start, unless the oldest waiting job has
+ * waited longer than `max_bypass_s`, in which case admission pauses until it
+ * fits.  Admitted jobs run concurrently, their kernels overlapping on the
+ * device; each job keeps its integrals, amplitudes and DIIS history resident
+ * until it converges.  Jobs share no mutable state, so a job's result does
+ * not depend on what else runs. */
+typedef struct aq_server aq_server;
+
+typedef struct aq_job_spec {
+    uint64_t seed;          /* synthetic system (aq_synth_host construction) */
+    int o, v, nQ;           /* nQ <= 0: 3*(o+v) */
+    double sigma;           /* <= 0: aq_synth_default_sigma */
+    aq_rhfccsd_config cfg;  /* solver options */
+} aq_job_spec;
+
+enum { AQ_JOB_QUEUED = 0, AQ_JOB_RUNNING = 1, AQ_JOB_DONE = 2, AQ_JOB_FAILED = 3 };
+
+typedef struct aq_job_result {
+    int64_t id;
+    int status;
+    int slot;
+    double t_submit, t_start, t_end;     /* seconds since the server started */
+    double energy, conv, mp2;
+    int iterations, converged;
+    int reduced_iterations;              /* iterations run below FP64 */
+    size_t bytes_reserved, bytes_peak;
+    int error_code;
+    char error[160];
+} aq_job_result;
+
+typedef struct aq_server_stats {
+    int64_t submitted, completed, failed, queued, running;
+    size_t budget, reserved;
+    double elapsed_s;
+} aq_server_stats;
+
+/* Conservative device bytes one job needs (integrals, amplitudes, W/F
+ * intermediates, DIIS history, operand copies and the integral build). */
+size_t aq_job_bytes_estimate(int o, int v, int nQ, int diis_order, int precision);
+
+int aq_server_create(int device, size_t hbm_budget, int slots, double max_bypass_s, aq_server** out);
+/* Queue a job (rejected with AQ_ERR_OOM if it can never fit the budget). */
+int aq_server_submit(aq_server* s, const aq_job_spec* spec, int64_t* job_id);
+int aq_server_poll(aq_server* s, int64_t job_id, aq_job_result* out);
+/* Block until the job is done or failed. */
+int aq_server_wait(aq_server* s, int64_t job_id, aq_job_result* out);
+/* Block until every submitted job is done or failed. */
+int aq_server_drain(aq_server* s);
+int aq_server_stats_get(aq_server* s, aq_server_stats* out);
+/* Drains, stops the workers and releases every context. */
+int aq_server_destroy(aq_server* s);
 
 #ifdef __cplusplus
 }

@@ -70,6 +70,19 @@ def test_host_generator_statistics():
         assert abs(z.mean()) < 0.02 and abs(z.std() - 1.0) < 0.02
 
 
+def test_job_bytes_estimate():
+    e = aq.job_bytes_estimate
+    # monotone in o and v, reduced precision needs its extra operand copies
+    assert e(4, 16) < e(5, 16) < e(5, 20)
+    assert e(20, 140, 0, 8, "bf16x3") > e(20, 140, 0, 8, "fp64")
+    assert e(20, 140, 0, 8) > e(20, 140, 0, 5)
+    # covers at least the resident FP64 integrals + DIIS history
+    o, v = 32, 256
+    floor = 8 * (v**4 + v**3 * o + 2 * 8 * o * o * v * v)
+    assert e(o, v, 0, 8) > floor
+    assert e(0, 10) == 0
+
+
 def test_no_device_is_a_clean_error_here():
     if os.path.exists("/dev/kfd") and os.environ.get("AQ_EXPECT_GPU"):
         pytest.skip("device present")
