@@ -1,0 +1,435 @@
+// Closed-shell SCF and the AO->MO step, plus the C ABI of the integral front
+// end.
+//
+// Reference: the localaoscf task (AOUHF<LocalUHF>, src/scf/aouhf.cxx and
+// src/scf/uhf.cxx:17-409; options aouhf.cxx:266-282) and the RHF AO->MO task
+// (src/operator/rhfaomoints.cxx:38-180).  A closed-shell UHF with equal alpha
+// and beta orbitals is RHF, which is what is solved here.  The Coulomb and
+// exchange builds J = (mn|ls) D_ls and K = (ml|ns) D_ls run on the device
+// through the contraction operator over the resident AO tensor; the n x n
+// algebra (orthogonalisation, DIIS, diagonalisation) stays on the host.  The
+// four quarter transformations of the AO->MO step are device contractions,
+// and the seven Dirac blocks <pq|rs> = (pr|qs) are cut from the MO tensor
+// with slice + permuted sum.
+#include "frontend.hpp"
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+using namespace aqamd;
+
+namespace {
+
+void ck(int rc) {
+    if (rc != AQ_OK) throw Error(rc, aq_last_error());
+}
+
+// RAII for tensors created here.
+struct Owned {
+    aq_tensor* t = nullptr;
+    Owned() = default;
+    Owned(aq_ctx* ctx, std::initializer_list<int64_t> len) {
+        std::vector<int64_t> l(len);
+        ck(aq_tensor_create(ctx, int(l.size()), l.data(), 0, &t));
+    }
+    ~Owned() {
+        if (t) aq_tensor_destroy(t);
+    }
+    Owned(const Owned&) = delete;
+    Owned& operator=(const Owned&) = delete;
+    aq_tensor* release() {
+        aq_tensor* r = t;
+        t = nullptr;
+        return r;
+    }
+};
+
+// C = A^T B (n x n row-major operands).
+void gemm_tn(int n, const double* A, const double* B, double* C) {
+    std::fill(C, C + size_t(n) * size_t(n), 0.0);
+    for (int k = 0; k < n; ++k)
+        for (int i = 0; i < n; ++i) {
+            const double a = A[size_t(k) * size_t(n) + size_t(i)];
+            if (a == 0.0) continue;
+            const double* b = B + size_t(k) * size_t(n);
+            double* c = C + size_t(i) * size_t(n);
+            for (int j = 0; j < n; ++j) c[j] += a * b[j];
+        }
+}
+
+// C = A B (n x n row-major).
+void gemm_nn(int n, const double* A, const double* B, double* C) {
+    std::fill(C, C + size_t(n) * size_t(n), 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < n; ++k) {
+            const double a = A[size_t(i) * size_t(n) + size_t(k)];
+            if (a == 0.0) continue;
+            const double* b = B + size_t(k) * size_t(n);
+            double* c = C + size_t(i) * size_t(n);
+            for (int j = 0; j < n; ++j) c[j] += a * b[j];
+        }
+}
+
+// Dense solve with partial pivoting; returns false on an exactly zero pivot.
+bool solve(int n, std::vector<double> A, std::vector<double>& b) {
+    for (int k = 0; k < n; ++k) {
+        int p = k;
+        for (int i = k + 1; i < n; ++i)
+            if (std::fabs(A[size_t(i) * size_t(n) + size_t(k)]) > std::fabs(A[size_t(p) * size_t(n) + size_t(k)])) p = i;
+        if (A[size_t(p) * size_t(n) + size_t(k)] == 0.0) return false;
+        if (p != k) {
+            for (int j = 0; j < n; ++j) std::swap(A[size_t(k) * size_t(n) + size_t(j)], A[size_t(p) * size_t(n) + size_t(j)]);
+            std::swap(b[size_t(k)], b[size_t(p)]);
+        }
+        for (int i = k + 1; i < n; ++i) {
+            const double f = A[size_t(i) * size_t(n) + size_t(k)] / A[size_t(k) * size_t(n) + size_t(k)];
+            for (int j = k; j < n; ++j) A[size_t(i) * size_t(n) + size_t(j)] -= f * A[size_t(k) * size_t(n) + size_t(j)];
+            b[size_t(i)] -= f * b[size_t(k)];
+        }
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double s = b[size_t(i)];
+        for (int j = i + 1; j < n; ++j) s -= A[size_t(i) * size_t(n) + size_t(j)] * b[size_t(j)];
+        b[size_t(i)] = s / A[size_t(i) * size_t(n) + size_t(i)];
+    }
+    return true;
+}
+
+}  // namespace
+
+namespace aqamd {
+namespace fe {
+
+// Cyclic Jacobi (Rutishauser's rotation: theta = (a_qq - a_pp) / (2 a_pq),
+// t = sgn(theta) / (|theta| + sqrt(theta^2 + 1))).  A is n x n symmetric
+// row-major; U receives the eigenvectors as columns of a row-major matrix.
+void eigh(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& U) {
+    const size_t N = size_t(n);
+    U.assign(N * N, 0.0);
+    for (size_t i = 0; i < N; ++i) U[i * N + i] = 1.0;
+    auto a = [&](size_t i, size_t j) -> double& { return A[i * N + j]; };
+    double scale = 0.0;
+    for (size_t i = 0; i < N * N; ++i) scale += A[i] * A[i];
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0;
+        for (size_t p = 0; p < N; ++p)
+            for (size_t q = p + 1; q < N; ++q) off += a(p, q) * a(p, q);
+        if (off <= 1e-32 * scale || off == 0.0) break;
+        for (size_t p = 0; p < N; ++p)
+            for (size_t q = p + 1; q < N; ++q) {
+                const double apq = a(p, q);
+                if (std::fabs(apq) < 1e-300) continue;
+                const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (size_t k = 0; k < N; ++k) {
+                    const double akp = a(k, p), akq = a(k, q);
+                    a(k, p) = c * akp - s * akq;
+                    a(k, q) = s * akp + c * akq;
+                }
+                for (size_t k = 0; k < N; ++k) {
+                    const double apk = a(p, k), aqk = a(q, k);
+                    a(p, k) = c * apk - s * aqk;
+                    a(q, k) = s * apk + c * aqk;
+                }
+                a(p, q) = a(q, p) = 0.0;
+                for (size_t k = 0; k < N; ++k) {
+                    const double ukp = U[k * N + p], ukq = U[k * N + q];
+                    U[k * N + p] = c * ukp - s * ukq;
+                    U[k * N + q] = s * ukp + c * ukq;
+                }
+            }
+    }
+    std::vector<size_t> order(N);
+    std::iota(order.begin(), order.end(), size_t(0));
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return a(x, x) < a(y, y); });
+    w.resize(N);
+    std::vector<double> V(N * N);
+    for (size_t k = 0; k < N; ++k) {
+        w[k] = a(order[k], order[k]);
+        for (size_t i = 0; i < N; ++i) V[i * N + k] = U[i * N + order[k]];
+    }
+    U.swap(V);
+}
+
+}  // namespace fe
+}  // namespace aqamd
+
+extern "C" {
+
+int aq_molecule_create(int natoms, const int* Z, const double* xyz, const char* genbas, int spherical, int charge,
+                       aq_molecule** out) {
+    return guarded([&] {
+        AQ_CHECK(out && Z && xyz && genbas && natoms >= 1, AQ_ERR_INVALID, "null argument or no atoms");
+        auto basis = fe::parse_genbas(genbas);
+        auto* m = new aq_molecule();
+        try {
+            m->d.Z.assign(Z, Z + natoms);
+            m->d.xyz.assign(xyz, xyz + 3 * size_t(natoms));
+            m->d.charge = charge;
+            m->d.spherical = spherical != 0;
+            fe::build_shells(m->d, basis);
+        } catch (...) {
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int aq_molecule_destroy(aq_molecule* mol) {
+    return guarded([&] { delete mol; });
+}
+
+int aq_molecule_info(const aq_molecule* mol, int* nao, int* nelectrons, int* nshells, double* nuclear_repulsion) {
+    return guarded([&] {
+        AQ_CHECK(mol, AQ_ERR_INVALID, "null molecule");
+        if (nao) *nao = mol->d.nao;
+        if (nelectrons) *nelectrons = mol->d.nelectrons();
+        if (nshells) *nshells = int(mol->d.shells.size());
+        if (nuclear_repulsion) *nuclear_repulsion = mol->d.nuclear_repulsion();
+    });
+}
+
+int aq_ints_1e(const aq_molecule* mol, double* S, double* T, double* V) {
+    return guarded([&] {
+        AQ_CHECK(mol && S && T && V, AQ_ERR_INVALID, "null argument");
+        fe::ints_1e(mol->d, S, T, V);
+    });
+}
+
+int aq_ints_2e(const aq_molecule* mol, int nthreads, double* eri) {
+    return guarded([&] {
+        AQ_CHECK(mol && eri, AQ_ERR_INVALID, "null argument");
+        fe::ints_2e(mol->d, nthreads, eri);
+    });
+}
+
+int aq_ints_2e_device(aq_ctx* ctx, const aq_molecule* mol, int nthreads, aq_tensor** out) {
+    return guarded([&] {
+        AQ_CHECK(ctx && mol && out, AQ_ERR_INVALID, "null argument");
+        const int64_t n = mol->d.nao;
+        std::vector<double> host(size_t(n) * size_t(n) * size_t(n) * size_t(n));
+        fe::ints_2e(mol->d, nthreads, host.data());
+        Owned e(ctx, {n, n, n, n});
+        ck(aq_write_all(ctx, e.t, host.data()));
+        ck(aq_ctx_sync(ctx));
+        *out = e.release();
+    });
+}
+
+int aq_scf_default_config(aq_scf_config* cfg) {
+    return guarded([&] {
+        AQ_CHECK(cfg, AQ_ERR_INVALID, "null config");
+        cfg->convergence = 1e-12;
+        cfg->max_iterations = 150;
+        cfg->diis_order = 6;
+    });
+}
+
+int aq_scf_rhf(aq_ctx* ctx, const aq_molecule* mol, const aq_tensor* eri, const aq_scf_config* cfg_in,
+               aq_scf_result* res, double* eps_out, double* C_out, double* F_out) {
+    return guarded([&] {
+        AQ_CHECK(ctx && mol && eri && res, AQ_ERR_INVALID, "null argument");
+        aq_scf_config cfg;
+        if (cfg_in)
+            cfg = *cfg_in;
+        else
+            aq_scf_default_config(&cfg);
+        const int n = mol->d.nao;
+        const int ne = mol->d.nelectrons();
+        AQ_CHECK(ne > 0 && ne % 2 == 0, AQ_ERR_UNSUPPORTED, "closed-shell SCF needs an even electron count");
+        const int nocc = ne / 2;
+        AQ_CHECK(nocc <= n, AQ_ERR_INVALID, "more occupied orbitals than basis functions");
+        AQ_CHECK(eri->ndim == 4 && eri->len[0] == n && eri->len[1] == n && eri->len[2] == n && eri->len[3] == n,
+                 AQ_ERR_INVALID, "AO tensor must be [nao,nao,nao,nao]");
+        const size_t N = size_t(n), NN = N * N;
+        std::vector<double> S(NN), T(NN), V(NN), H(NN);
+        fe::ints_1e(mol->d, S.data(), T.data(), V.data());
+        for (size_t i = 0; i < NN; ++i) H[i] = T[i] + V[i];
+        const double enuc = mol->d.nuclear_repulsion();
+
+        // X = S^{-1/2}
+        std::vector<double> A = S, s, U, X(NN), tmp(NN), tmp2(NN);
+        fe::eigh(n, A, s, U);
+        AQ_CHECK(s[0] > 1e-10, AQ_ERR_NUMERIC, "overlap matrix is singular to working precision");
+        for (size_t i = 0; i < N; ++i)
+            for (size_t j = 0; j < N; ++j) {
+                double x = 0.0;
+                for (size_t k = 0; k < N; ++k) x += U[i * N + k] * U[j * N + k] / std::sqrt(s[k]);
+                X[i * N + j] = x;
+            }
+        // C = X C' with F' = X^T F X = C' eps C'^T
+        std::vector<double> C(NN), eps(N);
+        auto diagonalise = [&](const std::vector<double>& F) {
+            gemm_tn(n, X.data(), F.data(), tmp.data());   // X^T F
+            gemm_nn(n, tmp.data(), X.data(), tmp2.data()); // X^T F X
+            std::vector<double> Cp;
+            fe::eigh(n, tmp2, eps, Cp);
+            gemm_nn(n, X.data(), Cp.data(), C.data());
+        };
+        diagonalise(H);
+
+        Owned Dd(ctx, {n, n}), Jd(ctx, {n, n}), Kd(ctx, {n, n});
+        std::vector<double> D(NN), J(NN), K(NN), F(NN), err(NN);
+        std::vector<std::vector<double>> fh, eh;
+        double E = 0.0, E_old = 0.0, conv = 0.0;
+        int it = 0;
+        bool done = false;
+        for (it = 1; it <= cfg.max_iterations; ++it) {
+            for (size_t i = 0; i < N; ++i)
+                for (size_t j = 0; j < N; ++j) {
+                    double d = 0.0;
+                    for (int k = 0; k < nocc; ++k) d += C[i * N + size_t(k)] * C[j * N + size_t(k)];
+                    D[i * N + j] = d;
+                }
+            // symmetric matrices: row- and column-major agree
+            ck(aq_write_all(ctx, Dd.t, D.data()));
+            ck(aq_contract(ctx, 1.0, eri, "mnls", Dd.t, "ls", 0.0, Jd.t, "mn"));
+            ck(aq_contract(ctx, 1.0, eri, "mlns", Dd.t, "ls", 0.0, Kd.t, "mn"));
+            ck(aq_read_all(ctx, Jd.t, J.data()));
+            ck(aq_read_all(ctx, Kd.t, K.data()));
+            E = enuc;
+            for (size_t i = 0; i < NN; ++i) {
+                F[i] = H[i] + 2.0 * J[i] - K[i];
+                E += D[i] * (H[i] + F[i]);
+            }
+            // orthonormal-basis commutator X^T (F D S - S D F) X (uhf.cxx:338-400)
+            gemm_nn(n, F.data(), D.data(), tmp.data());
+            gemm_nn(n, tmp.data(), S.data(), tmp2.data());
+            for (size_t i = 0; i < N; ++i)
+                for (size_t j = 0; j < N; ++j) err[i * N + j] = tmp2[i * N + j] - tmp2[j * N + i];
+            gemm_tn(n, X.data(), err.data(), tmp.data());
+            gemm_nn(n, tmp.data(), X.data(), err.data());
+            conv = 0.0;
+            for (double x : err) conv = std::max(conv, std::fabs(x));
+            if (it > 1 && std::fabs(E - E_old) < cfg.convergence && conv < 100.0 * cfg.convergence) {
+                done = true;
+                break;
+            }
+            E_old = E;
+            fh.push_back(F);
+            eh.push_back(err);
+            if (int(fh.size()) > std::max(cfg.diis_order, 1)) {
+                fh.erase(fh.begin());
+                eh.erase(eh.begin());
+            }
+            const int m = int(fh.size());
+            std::vector<double> Fx = F;
+            if (m >= 2) {
+                std::vector<double> B(size_t(m + 1) * size_t(m + 1), -1.0), rhs(size_t(m + 1), 0.0);
+                B[size_t(m) * size_t(m + 1) + size_t(m)] = 0.0;
+                for (int i = 0; i < m; ++i)
+                    for (int j = 0; j < m; ++j) {
+                        double b = 0.0;
+                        for (size_t k = 0; k < NN; ++k) b += eh[size_t(i)][k] * eh[size_t(j)][k];
+                        B[size_t(i) * size_t(m + 1) + size_t(j)] = b;
+                    }
+                rhs[size_t(m)] = -1.0;
+                if (solve(m + 1, B, rhs)) {
+                    std::fill(Fx.begin(), Fx.end(), 0.0);
+                    for (int i = 0; i < m; ++i)
+                        for (size_t k = 0; k < NN; ++k) Fx[k] += rhs[size_t(i)] * fh[size_t(i)][k];
+                }
+            }
+            diagonalise(Fx);
+        }
+        // canonical orbitals of the final Fock matrix
+        diagonalise(F);
+        res->energy = E;
+        res->conv = conv;
+        res->iterations = done ? it : cfg.max_iterations;
+        res->converged = done ? 1 : 0;
+        res->nocc = nocc;
+        if (eps_out) std::memcpy(eps_out, eps.data(), N * sizeof(double));
+        // column-major [nao, nmo]: C_out[m + p n] = C[m][p]
+        if (C_out)
+            for (size_t i = 0; i < N; ++i)
+                for (size_t p = 0; p < N; ++p) C_out[i + p * N] = C[i * N + p];
+        if (F_out) std::memcpy(F_out, F.data(), NN * sizeof(double));
+    });
+}
+
+int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* Ccm, const double* F, aq_tensor** out) {
+    return guarded([&] {
+        AQ_CHECK(ctx && eri && Ccm && F && out, AQ_ERR_INVALID, "null argument");
+        AQ_CHECK(eri->ndim == 4, AQ_ERR_INVALID, "AO tensor must be 4-dimensional");
+        const int64_t n = eri->len[0];
+        AQ_CHECK(eri->len[1] == n && eri->len[2] == n && eri->len[3] == n, AQ_ERR_INVALID, "AO tensor must be [n,n,n,n]");
+        AQ_CHECK(nocc >= 1 && nocc < n, AQ_ERR_INVALID, "need 1 <= nocc < nao");
+        const int64_t o = nocc, v = n - nocc;
+        const size_t N = size_t(n);
+        // Fock matrix in the MO basis, f = C^T F C
+        std::vector<double> CF(N * N, 0.0), f(N * N, 0.0);
+        for (size_t p = 0; p < N; ++p)
+            for (size_t l = 0; l < N; ++l) {
+                double s = 0.0;
+                for (size_t m = 0; m < N; ++m) s += Ccm[m + p * N] * F[m * N + l];
+                CF[p * N + l] = s;
+            }
+        for (size_t p = 0; p < N; ++p)
+            for (size_t q = 0; q < N; ++q) {
+                double s = 0.0;
+                for (size_t l = 0; l < N; ++l) s += CF[p * N + l] * Ccm[l + q * N];
+                f[p * N + q] = s;
+            }
+        Owned fAI(ctx, {v, o}), fAB(ctx, {v, v}), fIJ(ctx, {o, o});
+        {
+            std::vector<double> h(size_t(v * o));
+            for (int64_t i = 0; i < o; ++i)
+                for (int64_t a = 0; a < v; ++a) h[size_t(a + i * v)] = f[size_t(o + a) * N + size_t(i)];
+            ck(aq_write_all(ctx, fAI.t, h.data()));
+            h.assign(size_t(v * v), 0.0);
+            for (int64_t b = 0; b < v; ++b)
+                for (int64_t a = 0; a < v; ++a) h[size_t(a + b * v)] = f[size_t(o + a) * N + size_t(o + b)];
+            ck(aq_write_all(ctx, fAB.t, h.data()));
+            h.assign(size_t(o * o), 0.0);
+            for (int64_t j = 0; j < o; ++j)
+                for (int64_t i = 0; i < o; ++i) h[size_t(i + j * o)] = f[size_t(i) * N + size_t(j)];
+            ck(aq_write_all(ctx, fIJ.t, h.data()));
+        }
+        // four quarter transformations
+        Owned Cd(ctx, {n, n});
+        ck(aq_write_all(ctx, Cd.t, Ccm));
+        Owned mo(ctx, {n, n, n, n});
+        {
+            Owned x1(ctx, {n, n, n, n});
+            ck(aq_contract(ctx, 1.0, Cd.t, "mp", eri, "mnls", 0.0, x1.t, "pnls"));
+            ck(aq_contract(ctx, 1.0, Cd.t, "nq", x1.t, "pnls", 0.0, mo.t, "pqls"));
+            ck(aq_contract(ctx, 1.0, Cd.t, "lr", mo.t, "pqls", 0.0, x1.t, "pqrs"));
+            ck(aq_contract(ctx, 1.0, Cd.t, "st", x1.t, "pqrs", 0.0, mo.t, "pqrt"));
+        }
+        ck(aq_ctx_trim(ctx));
+        // <pq|rs> = (pr|qs): cut the box [p-range, r-range, q-range, s-range]
+        // out of the MO tensor, then permute it into [p, q, r, s].
+        auto block = [&](int64_t r0, int64_t r1, int64_t r2, int64_t r3, Owned& dst) {
+            // dst index order p q r s; source box order p r q s
+            const int64_t lp = r0 == 0 ? o : v, lq = r1 == 0 ? o : v, lr = r2 == 0 ? o : v, ls = r3 == 0 ? o : v;
+            const int64_t off[2] = {0, o};
+            Owned box(ctx, {lp, lr, lq, ls});
+            const int64_t startA[4] = {off[r0 != 0], off[r2 != 0], off[r1 != 0], off[r3 != 0]};
+            const int64_t startB[4] = {0, 0, 0, 0};
+            const int64_t ext[4] = {lp, lr, lq, ls};
+            ck(aq_slice(ctx, 1.0, mo.t, startA, 0.0, box.t, startB, ext));
+            ck(aq_tensor_create(ctx, 4, std::vector<int64_t>{lp, lq, lr, ls}.data(), 0, &dst.t));
+            ck(aq_sum(ctx, 1.0, box.t, "prqs", 0.0, dst.t, "pqrs"));
+        };
+        const int V_ = 1, O_ = 0;
+        Owned VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL;
+        block(V_, V_, V_, V_, VABCD);
+        block(V_, V_, V_, O_, VABCI);
+        block(V_, V_, O_, O_, VABIJ);
+        block(V_, O_, V_, O_, VAIBJ);
+        block(V_, O_, O_, V_, VAIJB);
+        block(V_, O_, O_, O_, VAIJK);
+        block(O_, O_, O_, O_, VIJKL);
+        ck(aq_ctx_sync(ctx));
+        Owned* all[10] = {&fAI, &fAB, &fIJ, &VABCD, &VABCI, &VABIJ, &VAIBJ, &VAIJB, &VAIJK, &VIJKL};
+        for (int k = 0; k < 10; ++k) out[k] = all[k]->release();
+    });
+}
+
+}  // extern "C"

@@ -248,6 +248,69 @@ double aq_synth_default_sigma(int v, int nQ);
 int aq_synth_build(aq_ctx* ctx, int o, int v, int nQ, const double* eps, const double* B,
                    aq_tensor** out);
 
+/* ---- integral front end for real molecules ------------------------------
+ *
+ * The reference builds the CCSD input with the tasks 1eints, 2eints,
+ * localaoscf and aomoints (test/input.aq:12-17).  These entry points replace
+ * them for closed shells: the molecule + basis, the AO integrals, RHF SCF
+ * (J/K builds as device contractions) and the AO->MO step that yields the
+ * ten tensors aq_rhfccsd_create takes. */
+typedef struct aq_molecule aq_molecule;
+
+/* Molecule with a GENBAS-format basis set: BasisSet::readBasisSet
+ * (src/input/basis.cxx:15-75), Shell normalisation (src/integrals/shell.cxx:
+ * 114-160), Molecule (src/input/molecule.cxx:180-182, 1237-1275).  Z[natoms],
+ * xyz[3*natoms] in bohr (the reference converts angstrom with 1 A =
+ * 1.88972612456506198632428439 bohr, molecule.cxx:1335-1336); `genbas` is the
+ * text of a GENBAS file holding an entry for every element present;
+ * spherical != 0 selects pure harmonics (the reference default,
+ * molecule.cxx:1364-1367).  Each contraction of a generally contracted shell
+ * becomes its own segmented shell. */
+int aq_molecule_create(int natoms, const int* Z, const double* xyz, const char* genbas, int spherical,
+                       int charge, aq_molecule** out);
+int aq_molecule_destroy(aq_molecule* mol);
+int aq_molecule_info(const aq_molecule* mol, int* nao, int* nelectrons, int* nshells,
+                     double* nuclear_repulsion);
+/* 1eints (src/integrals/1eints.cxx): overlap S, kinetic T and nuclear
+ * attraction V, nao*nao each (symmetric), computed on the host. */
+int aq_ints_1e(const aq_molecule* mol, double* S, double* T, double* V);
+/* 2eints (src/integrals/2eints.cxx, os.cxx): eri[m + n*N + l*N^2 + s*N^3] =
+ * (mn|ls) for all nao^4 elements, on the host with `nthreads` threads
+ * (<= 0: every core).  aq_ints_2e_device uploads the same tensor to a new
+ * [nao,nao,nao,nao] device tensor. */
+int aq_ints_2e(const aq_molecule* mol, int nthreads, double* eri);
+int aq_ints_2e_device(aq_ctx* ctx, const aq_molecule* mol, int nthreads, aq_tensor** out);
+
+/* localaoscf (src/scf/aouhf.cxx, uhf.cxx:17-409) for closed shells.
+ * Defaults (aouhf.cxx:266-282): convergence 1e-12 (energy change; the
+ * orthonormal-basis commutator must also fall below 100x that), 150
+ * iterations, DIIS order 6.  frozen_core is false there and here. */
+typedef struct aq_scf_config {
+    double convergence;
+    int max_iterations;
+    int diis_order;
+} aq_scf_config;
+typedef struct aq_scf_result {
+    double energy;      /* total SCF energy including nuclear repulsion */
+    double conv;        /* max |X^T (FDS - SDF) X| of the last iteration */
+    int iterations;
+    int converged;
+    int nocc;
+} aq_scf_result;
+int aq_scf_default_config(aq_scf_config* cfg);
+/* eri: the device AO tensor of aq_ints_2e_device.  Outputs (optional):
+ * eps[nao] orbital energies ascending, C[nao*nao] column-major MO
+ * coefficients (C[m + p*nao]), F[nao*nao] the final AO Fock matrix. */
+int aq_scf_rhf(aq_ctx* ctx, const aq_molecule* mol, const aq_tensor* eri, const aq_scf_config* cfg,
+               aq_scf_result* res, double* eps, double* C, double* F);
+/* aomoints for RHF (src/operator/rhfaomoints.cxx:38-180): the MO Fock blocks
+ * f = C^T F C and the seven Dirac blocks <pq|rs> = (pr|qs) over the occupied
+ * (first nocc) and virtual MOs.  Creates 10 tensors in the order of
+ * aq_synth_build: fAI, fAB, fIJ, VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK,
+ * VIJKL. */
+int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* C, const double* F,
+                aq_tensor** out);
+
 /* ---- serving: continuous batching of CCSD jobs (no reference counterpart;
  * Aquarius runs one task DAG sequentially, src/task/task.cxx:495-588) -------
  *

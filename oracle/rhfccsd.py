@@ -30,7 +30,10 @@ BLOCKS = ("VABCD", "VABCI", "VABIJ", "VAIBJ", "VAIJB", "VAIJK", "VIJKL")
 def default_config():
     """Task spec defaults (rhfccsd.cxx:332-353)."""
     return {"convergence": 1e-9, "max_iterations": 50, "conv_type": "MAXE",
-            "diis": {"damping": 0.0, "start": 1, "order": 5, "jacobi": False}}
+            "diis": {"damping": 0.0, "start": 1, "order": 5, "jacobi": False},
+            # not a reference option: True holds T1 at zero, which turns the
+            # equations into CCD (the check against test/input.aq:22)
+            "freeze_t1": False}
 
 
 class Denominator:
@@ -178,6 +181,8 @@ class RHFCCSD:
 
         self.T1[...] = self.fAI                                   # :79
         ops.weight(self.T1, [dA, dI])                             # :80
+        if self.config.get("freeze_t1"):
+            self.T1[...] = 0.0      # CCD: the CCSD equations restricted to T1 = 0
         self.T2[...] = V["VABIJ"]                                 # :82
         ops.weight(self.T2, [dA, dA, dI, dI])                     # :83
 
@@ -277,6 +282,8 @@ class RHFCCSD:
 
         ops.weight(Z1, [dA, dI])                                          # :313
         ops.weight(Z2, [dA, dA, dI, dI])                                  # :314
+        if self.config.get("freeze_t1"):
+            Z1[...] = 0.0
         T1 += Z1                                                          # :316
         T2 += Z2                                                          # :317
 
