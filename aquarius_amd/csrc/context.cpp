@@ -31,6 +31,19 @@ void dfree(aq_ctx* ctx, void* p) {
     if (p) (void)hipFreeAsync(p, ctx->stream);
 }
 
+// Return the memory the device's default pool holds but no allocation uses
+// (blocks whose stream-ordered frees have completed, on any stream) to the
+// driver.  The pool keeps freed blocks between synchronisations (release
+// threshold at its maximum), which is what makes repeated allocation cheap
+// within a job; across jobs of different sizes those cached blocks can be
+// the wrong shapes for the next request, so they are trimmed when a job
+// ends and before an allocation is retried.
+static void trim_device_pool(aq_ctx* ctx) {
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess) (void)hipMemPoolTrimTo(pool, 0);
+    (void)hipGetLastError();
+}
+
 static void note_peak(aq_ctx* ctx) { ctx->peak_bytes = std::max(ctx->peak_bytes, ctx_in_use(ctx)); }
 
 static void pool_release_all(aq_ctx* ctx) {
@@ -70,6 +83,7 @@ static void* raw_alloc(aq_ctx* ctx, size_t bytes) {
         pool_release_all(ctx);
         evict_cache(ctx, ctx->budget);   // drop every cached copy
         AQ_HIP(hipStreamSynchronize(ctx->stream));
+        trim_device_pool(ctx);
         p = dmalloc(ctx, bytes);
         if (!p) throw Error(AQ_ERR_OOM, "device allocation of " + std::to_string(bytes) + " B failed");
     }
@@ -283,6 +297,8 @@ int aq_ctx_trim(aq_ctx* ctx) {
         ctx->cache.clear();
         ctx->cache_lru.clear();
         ctx->cache_bytes = 0;
+        AQ_HIP(hipStreamSynchronize(ctx->stream));
+        trim_device_pool(ctx);
     });
 }
 

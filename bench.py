@@ -123,7 +123,10 @@ def serving(a):
            "config": {"workload": f"{a.jobs} synthetic CCSD jobs, o~U{{{a.o_min}..{a.o_max}}}, v=round(o*U[4,8]), "
                                   f"Poisson arrivals {a.rate}/s, HBM budget {a.budget_gb} GB, {a.slots} slots",
                       "seed": a.seed, "convergence": a.convergence, "diis_order": a.diis},
-           "jobs": len(res), "converged": len(ok), "wall_s": wall,
+           "jobs": len(res), "converged": len(ok),
+           "failed": sum(r["status"] == aq.JOB_FAILED for r in res),
+           "first_error": next((r["error"] for r in res if r["status"] == aq.JOB_FAILED), ""),
+           "wall_s": wall,
            "time_to_E_s": {"p50": float(np.percentile(tte, 50)), "p99": float(np.percentile(tte, 99)),
                            "max": float(tte.max()), "mean": float(tte.mean())}}
     # one-at-a-time FP64 solves of a sample of the same stream (no arrivals: back to back)

@@ -84,7 +84,8 @@ int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
 int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget);
 /* High-water mark of the bytes held since creation or the last reset. */
 int aq_ctx_peak_memory(aq_ctx* ctx, size_t* peak, int reset);
-/* Drop cached operand copies and pooled scratch (keeps tensors). */
+/* Drop cached operand copies and pooled scratch (keeps tensors), then return
+ * the device pool's unused memory to the driver. */
 int aq_ctx_trim(aq_ctx* ctx);
 /* Per-kernel-class timing with HIP events on the context's stream.
  * When enabled every GEMM / elementwise launch is bracketed by events. */
