@@ -36,6 +36,12 @@ class RHFCCSDConfig(C.Structure):
                 ("diis_reset_on_refine", C.c_int), ("fp32_stage_below", C.c_double)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("trans_a", C.c_int), ("trans_b", C.c_int),
+                ("m_bucket", C.c_int), ("n_bucket", C.c_int), ("k_bucket", C.c_int), ("batch_bucket", C.c_int),
+                ("variant", C.c_int), ("ms", C.c_float * 3)]
+
+
 ITER_CB = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)
 
 _P = C.c_void_p
@@ -52,6 +58,8 @@ _SIGS = {
     "aq_ctx_set_precision": [_P, C.c_int, C.c_double],
     "aq_ctx_get_precision": [_P, C.POINTER(C.c_int)],
     "aq_ctx_set_gemm_variant": [_P, C.c_int, C.c_int],
+    "aq_ctx_set_autotune": [_P, C.c_int],
+    "aq_autotune_table": [C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)],
     "aq_ctx_memory": [_P, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)],
     "aq_ctx_trim": [_P],
     "aq_ctx_enable_timing": [_P, C.c_int],
@@ -85,6 +93,7 @@ _SIGS = {
     "aq_rhfccsd_iterate": [_P, _DP, _DP],
     "aq_rhfccsd_run": [_P, ITER_CB, C.c_void_p, _DP, _DP, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "aq_rhfccsd_tensor": [_P, C.c_int, C.POINTER(_P)],
+    "aq_rhfccsd_hbar": [_P, C.c_int],
     "aq_rhfccsd_current_precision": [_P, C.POINTER(C.c_int)],
     "aq_synth_host": [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _DP, _DP],
     "aq_synth_default_sigma": [C.c_int, C.c_int],
@@ -146,6 +155,9 @@ class Context:
 
     def set_gemm_variant(self, prec, variant):
         _ck(lib().aq_ctx_set_gemm_variant(self.h, int(PRECISIONS.get(prec, prec)), int(variant)))
+
+    def set_autotune(self, enable=True):
+        _ck(lib().aq_ctx_set_autotune(self.h, 1 if enable else 0))
 
     def memory(self):
         a, b = C.c_size_t(), C.c_size_t()
@@ -272,6 +284,16 @@ class Tensor:
         _ck(lib().aq_write(self.ctx.h, self.h, len(keys), keys.ctypes.data_as(_I64P), _dp(vals), alpha, beta))
 
 
+def autotune_table():
+    """The GEMM tuning table: one dict per shape bucket."""
+    n = C.c_int()
+    _ck(lib().aq_autotune_table(None, 0, C.byref(n)))
+    arr = (TuneEntry * max(n.value, 1))()
+    _ck(lib().aq_autotune_table(arr, n.value, C.byref(n)))
+    return [{f: (list(getattr(e, f)) if f == "ms" else getattr(e, f)) for f, _ in TuneEntry._fields_}
+            for e in arr[:n.value]]
+
+
 def default_config(**over):
     cfg = RHFCCSDConfig()
     _ck(lib().aq_rhfccsd_default_config(C.byref(cfg)))
@@ -285,7 +307,7 @@ def default_config(**over):
 FOCK = ("fAI", "fAB", "fIJ")
 BLOCKS = ("VABCD", "VABCI", "VABIJ", "VAIBJ", "VAIJB", "VAIJK", "VIJKL")
 STATE = {"T1": 0, "T2": 1, "Z1": 2, "Z2": 3, "Tau": 4, "FAE": 10, "FMI": 11, "FME": 12, "WMNIJ": 13,
-         "WMNEJ": 14, "WAMIJ": 15, "WAMEI": 16, "WAMIE": 17}
+         "WMNEJ": 14, "WAMIJ": 15, "WAMEI": 16, "WAMIE": 17, "WABEJ": 18}
 
 
 class Integrals:
@@ -376,6 +398,10 @@ class RHFCCSD:
         e, c, n, ok = C.c_double(), C.c_double(), C.c_int(), C.c_int()
         _ck(lib().aq_rhfccsd_run(self.h, cb, None, C.byref(e), C.byref(c), C.byref(n), C.byref(ok)))
         return {"energy": e.value, "conv": c.value, "iterations": n.value, "converged": bool(ok.value)}
+
+    def hbar(self, with_wabej=False):
+        """Form the Hbar products of the current amplitudes (rhfccsd.cxx:108-188)."""
+        _ck(lib().aq_rhfccsd_hbar(self.h, 1 if with_wabej else 0))
 
     def tensor(self, name):
         p = _P()

@@ -254,6 +254,20 @@ int aq_ctx_set_precision(aq_ctx* ctx, int precision, double min_reduced_flops) {
     });
 }
 
+int aq_ctx_set_autotune(aq_ctx* ctx, int enable) {
+    return guarded([&] {
+        AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
+        ctx->autotune = enable != 0;
+    });
+}
+
+int aq_autotune_table(aq_tune_entry* out, int cap, int* n) {
+    return guarded([&] {
+        AQ_CHECK(n && cap >= 0, AQ_ERR_INVALID, "null count");
+        *n = autotune_entries(out, cap);
+    });
+}
+
 int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant) {
     return guarded([&] {
         AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");

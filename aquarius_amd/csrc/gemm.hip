@@ -25,8 +25,12 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
+#include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 
 namespace aqamd {
 
@@ -414,14 +418,128 @@ static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid, int variant) {
 // single wave per SIMD exposes (FP64 35 -> 56 TF/s; FP32 unchanged at ~108).
 int default_gemm_variant(Fmt fmt) { return fmt == FMT_BF16X2 ? 0 : 2; }
 
-int gemm_kchunk_bk(Fmt fmt, int BM, int BN, int variant) {
-    if (fmt == FMT_BF16X2) return 32;
-    return (BM == 128 && BN == 128 && variant == 1) ? 32 : 16;
-}
-
 template <int BM, int BN, int WM, int WN>
 static void run_cfg_bf16(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
     hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN>), grid, dim3(256), 0, ctx->stream, a);
+}
+
+// Kernel launch for one (format, block shape, variant); the caller checked the shape.
+static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const GemmArgs& a, dim3 grid) {
+    if (fmt == FMT_F64) {
+        if (BM == 128 && BN == 128) run_cfg<double, 128, 128, 2, 2>(ctx, a, grid, variant);
+        else if (BM == 64) run_cfg<double, 64, 64, 2, 2>(ctx, a, grid, 0);
+        else if (BN == 16) run_cfg<double, 128, 16, 4, 1>(ctx, a, grid, 0);
+        else run_cfg<double, 16, 128, 1, 4>(ctx, a, grid, 0);
+    } else if (fmt == FMT_F32) {
+        if (BM == 128 && BN == 128) run_cfg<float, 128, 128, 2, 2>(ctx, a, grid, variant);
+        else if (BM == 64) run_cfg<float, 64, 64, 2, 2>(ctx, a, grid, 0);
+        else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
+        else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
+    } else {
+        if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
+        else if (BM == 256) run_cfg_bf16<256, 128, 2, 2>(ctx, a, grid);
+        else if (BN == 256) run_cfg_bf16<128, 256, 2, 2>(ctx, a, grid);
+        else if (BM == 64) run_cfg_bf16<64, 64, 2, 2>(ctx, a, grid);
+        else if (BN == 16) run_cfg_bf16<128, 16, 4, 1>(ctx, a, grid);
+        else run_cfg_bf16<16, 128, 1, 4>(ctx, a, grid);
+    }
+}
+
+// Block shape of a variant: BF16x3 variants 1 and 2 enlarge the 128 x 128
+// block to 256 x 128 / 128 x 256 (128 x 64 / 64 x 128 per wave), which reads a
+// third less LDS per MFMA; the FP64/FP32 variants change BK and occupancy only.
+static void variant_shape(Fmt fmt, int variant, int64_t M, int64_t N, int& BM, int& BN) {
+    if (fmt == FMT_BF16X2 && BM == 128 && BN == 128) {
+        if (variant == 1 && M >= 256) BM = 256;
+        else if (variant == 2 && N >= 256) BN = 256;
+    }
+}
+
+// ------------------------------------------------------------- autotuner
+//
+// Per device, one table from a shape bucket (format, operand layouts, M, N, K
+// and batch in half-octaves) to the variant that ran fastest when the bucket
+// was first met.  A context with autotune on times every candidate variant
+// once on that first GEMM (HIP events on its stream, into a scratch output),
+// records the winner and reuses it for every later GEMM of the bucket, from
+// any context.  Variants never change results (the split-K partition does not
+// depend on them), so tuning affects speed only.
+namespace {
+
+struct TuneKey {
+    int device, fmt, ta, tb, m, n, k, batch;
+    bool operator<(const TuneKey& o) const {
+        return std::tie(device, fmt, ta, tb, m, n, k, batch) < std::tie(o.device, o.fmt, o.ta, o.tb, o.m, o.n, o.k, o.batch);
+    }
+};
+
+struct TuneEntry {
+    int variant = -1;
+    float ms[3] = {-1.f, -1.f, -1.f};
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneEntry> g_tune;
+
+int half_octave(int64_t x) { return int(std::lround(2.0 * std::log2(double(std::max<int64_t>(x, 1))))); }
+
+}  // namespace
+
+static int tuned_variant(aq_ctx* ctx, const GemmDesc& d, int BM, int BN, const GemmArgs& a) {
+    const int defv = default_gemm_variant(d.fmt);
+    if (!(BM == 128 && BN == 128)) return defv;             // only the large tile has variants
+    TuneKey key{ctx->device, int(d.fmt), d.transA, d.transB, half_octave(d.M), half_octave(d.N), half_octave(d.K),
+                half_octave(d.batch)};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) return it->second.variant >= 0 ? it->second.variant : defv;
+    }
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return defv;
+    // trial output (and partials) so that C is untouched
+    const size_t cbytes = size_t(a.M) * size_t(a.N) * size_t(a.batch) * 8;
+    const size_t pbytes = a.nsplit > 1 ? size_t(a.nsplit) * cbytes : 0;
+    if (ctx_in_use(ctx) + cbytes + pbytes > ctx->budget) return defv;
+    TuneEntry e;
+    {
+        Scratch tc(ctx, cbytes), tp(ctx, pbytes);
+        GemmArgs t = a;
+        t.C = tc.as<double>();
+        t.ldc = a.M;
+        t.strideC = a.M * a.N;
+        t.beta = 0.0;
+        t.P = a.nsplit > 1 ? tp.as<double>() : nullptr;
+        hipEvent_t e0, e1;
+        AQ_HIP(hipEventCreate(&e0));
+        AQ_HIP(hipEventCreate(&e1));
+        float best = 0.f;
+        for (int v = 0; v < 3; ++v) {
+            int bm = BM, bn = BN;
+            variant_shape(d.fmt, v, d.M, d.N, bm, bn);
+            if (d.fmt == FMT_BF16X2 && v > 0 && bm == BM && bn == BN) continue;   // same kernel as variant 0
+            const int64_t tiles = ((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn);
+            dim3 grid(unsigned(tiles), 1, unsigned(a.batch * a.nsplit));
+            dispatch(ctx, d.fmt, bm, bn, v, t, grid);           // warm
+            AQ_HIP(hipEventRecord(e0, ctx->stream));
+            dispatch(ctx, d.fmt, bm, bn, v, t, grid);
+            AQ_HIP(hipEventRecord(e1, ctx->stream));
+            AQ_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            AQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            e.ms[v] = ms;
+            if (e.variant < 0 || ms < best) {
+                best = ms;
+                e.variant = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        AQ_HIP(hipGetLastError());
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.emplace(key, e);
+    return e.variant >= 0 ? e.variant : defv;
 }
 
 void gemm(aq_ctx* ctx, const GemmDesc& d) {
@@ -432,28 +550,30 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
                      d.planeB % 8 == 0 && d.strideA % 8 == 0 && d.strideB % 8 == 0,
                  AQ_ERR_INVALID, "BF16x3 GEMM needs K-contiguous operands with 8-aligned leading dimensions");
 
-    // tile configuration
+    // base block shape from the problem shape
     int BM = 128, BN = 128;
     if (d.N <= 16) { BM = 128; BN = 16; }
     else if (d.M <= 16) { BM = 16; BN = 128; }
     else if (((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch < 256) { BM = 64; BN = 64; }
-    const int64_t tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
-    AQ_CHECK(tiles < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "GEMM grid too large");
 
-    const int variant = ctx->gemm_variant[d.fmt] >= 0 ? ctx->gemm_variant[d.fmt] : default_gemm_variant(d.fmt);
-    // split-K when the output alone cannot fill the device
-    const int BK = gemm_kchunk_bk(d.fmt, BM, BN, variant);
+    // Split-K when the output alone cannot fill the device.  The decision and
+    // the k-partition depend only on the shape (base tile grid, 32-wide
+    // k-steps that every variant's BK divides), so every kernel variant
+    // computes each element with the same summation order.
+    constexpr int64_t KSTEP = 32;
+    const int64_t base_tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
+    AQ_CHECK(base_tiles < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "GEMM grid too large");
     int nsplit = 1;
-    const int64_t ktiles = (d.K + BK - 1) / BK;
+    const int64_t ksteps = (d.K + KSTEP - 1) / KSTEP;
     const int64_t want = 1024;
-    if (tiles * d.batch < 512 && ktiles >= 16) {
-        int64_t s = (want + tiles * d.batch - 1) / (tiles * d.batch);
-        s = std::min<int64_t>(s, ktiles / 8);
+    if (base_tiles * d.batch < 512 && ksteps >= 16) {
+        int64_t s = (want + base_tiles * d.batch - 1) / (base_tiles * d.batch);
+        s = std::min<int64_t>(s, ksteps / 8);
         s = std::min<int64_t>(s, 256);
         nsplit = int(std::max<int64_t>(s, 1));
         while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
     }
-    int64_t kchunk = ((ktiles + nsplit - 1) / nsplit) * BK;
+    int64_t kchunk = ((ksteps + nsplit - 1) / nsplit) * KSTEP;
     nsplit = int((d.K + kchunk - 1) / kchunk);
 
     GemmArgs a;
@@ -465,6 +585,11 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     a.batch = int(d.batch);
     a.transA = d.transA; a.transB = d.transB;
     a.P = nullptr;
+
+    int variant = ctx->gemm_variant[d.fmt];
+    if (variant < 0) variant = ctx->autotune ? tuned_variant(ctx, d, BM, BN, a) : default_gemm_variant(d.fmt);
+    variant_shape(d.fmt, variant, d.M, d.N, BM, BN);
+    const int64_t tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
 
     std::unique_ptr<Scratch> part;
     if (nsplit > 1) {
@@ -482,26 +607,33 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
              d.transB ? 'T' : 'N');
     {
         TimedLaunch tl(ctx, name, flops, bytes);
-        if (d.fmt == FMT_F64) {
-            if (BM == 128 && BN == 128) run_cfg<double, 128, 128, 2, 2>(ctx, a, grid, variant);
-            else if (BM == 64) run_cfg<double, 64, 64, 2, 2>(ctx, a, grid, 0);
-            else if (BN == 16) run_cfg<double, 128, 16, 4, 1>(ctx, a, grid, 0);
-            else run_cfg<double, 16, 128, 1, 4>(ctx, a, grid, 0);
-        } else if (d.fmt == FMT_F32) {
-            if (BM == 128 && BN == 128) run_cfg<float, 128, 128, 2, 2>(ctx, a, grid, variant);
-            else if (BM == 64) run_cfg<float, 64, 64, 2, 2>(ctx, a, grid, 0);
-            else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
-            else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
-        } else {
-            if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
-            else if (BM == 64) run_cfg_bf16<64, 64, 2, 2>(ctx, a, grid);
-            else if (BN == 16) run_cfg_bf16<128, 16, 4, 1>(ctx, a, grid);
-            else run_cfg_bf16<16, 128, 1, 4>(ctx, a, grid);
-        }
+        dispatch(ctx, d.fmt, BM, BN, variant, a, grid);
         AQ_HIP(hipGetLastError());
     }
     if (nsplit > 1)
         launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
+}
+
+int autotune_entries(aq_tune_entry* out, int cap) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    int n = 0;
+    for (const auto& kv : g_tune) {
+        if (out && n < cap) {
+            aq_tune_entry& e = out[n];
+            e.device = kv.first.device;
+            e.precision = kv.first.fmt == FMT_F64 ? AQ_PREC_FP64 : kv.first.fmt == FMT_F32 ? AQ_PREC_FP32 : AQ_PREC_BF16X3;
+            e.trans_a = kv.first.ta;
+            e.trans_b = kv.first.tb;
+            e.m_bucket = kv.first.m;
+            e.n_bucket = kv.first.n;
+            e.k_bucket = kv.first.k;
+            e.batch_bucket = kv.first.batch;
+            e.variant = kv.second.variant;
+            for (int v = 0; v < 3; ++v) e.ms[v] = kv.second.ms[v];
+        }
+        ++n;
+    }
+    return n;
 }
 
 }  // namespace aqamd

@@ -106,6 +106,7 @@ struct aq_ctx {
     int precision = AQ_PREC_FP64;
     double min_reduced_flops = 0.0;
     int gemm_variant[3] = {-1, -1, -1};   // per format; -1 = library default
+    bool autotune = false;                // pick GEMM variants per shape bucket by timing
     uint64_t next_id = 1;
 
     // scratch pool: exact-size free lists
@@ -224,6 +225,7 @@ struct GemmDesc {
     double* C; int64_t ldc; int64_t strideC;
     const char* tag;   // timing class name
 };
+int autotune_entries(aq_tune_entry* out, int cap);   // gemm.hip
 void gemm(aq_ctx* ctx, const GemmDesc& d);
 
 }  // namespace aqamd

@@ -202,6 +202,7 @@ struct aq_rhfccsd {
     Tensor VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL;
     Tensor T1, T2, Z1, Z2, Tau, T2SA, TauSA, VABIJSA, VAIJBSA;
     Tensor FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE;
+    Tensor WABEJ;                       // formed by hbar(true) only
     std::vector<double> dA, dI;
     Tensor d_dA, d_dI;
     std::unique_ptr<DIIS> diis;
@@ -379,6 +380,76 @@ struct aq_rhfccsd {
         diis->extrapolate({&T1, &T2}, {&Z1, &Z2});                 // :325-326
         ++iter;
     }
+
+    // Hbar products of the current amplitudes (rhfccsd.cxx:108-188), in FP64.
+    void hbar(bool with_wabej) {
+        tensor::check(aq_ctx_set_precision(ctx, AQ_PREC_FP64, 0.0));
+        Tau["abij"] = T2["abij"];                                  // :119
+        Tau["abij"] += T1["ai"] * T1["bj"];                        // :120
+        T2SA["abij"] = 2 * T2["abij"];                             // :122
+        T2SA["abij"] -= T2["abji"];                                // :123
+
+        FME["me"] = fAI["em"];                                     // :125
+        FME["me"] += VABIJSA["efmn"] * T1["fn"];                   // :126
+
+        FMI["mi"] = fIJ["mi"];                                     // :128
+        FMI["mi"] += VABIJSA["efmn"] * T2["efin"];                 // :129
+        FMI["mi"] += FME["me"] * T1["ei"];                         // :130
+        FMI["mi"] += 2 * VAIJK["finm"] * T1["fn"];                 // :131
+        FMI["mi"] -= VAIJK["fimn"] * T1["fn"];                     // :132
+
+        FAE["ae"] = fAB["ae"];                                     // :134
+        FAE["ae"] -= VABIJSA["efmn"] * T2["afmn"];                 // :135
+        FAE["ae"] -= FME["me"] * T1["am"];                         // :136
+        FAE["ae"] += 2 * VABCI["efam"] * T1["fm"];                 // :137
+        FAE["ae"] -= VABCI["feam"] * T1["fm"];                     // :138
+
+        WMNIJ["mnij"] = 0.5 * VIJKL["mnij"];                       // :140
+        WMNIJ["mnij"] += 0.5 * VABIJ["efmn"] * Tau["efij"];        // :141
+        WMNIJ["mnij"] += VAIJK["ejmn"] * T1["ei"];                 // :142
+        WMNIJ["mnij"] += WMNIJ["nmji"];                            // :143 (aliased)
+
+        WMNEJ["mnej"] = VAIJK["ejmn"];                             // :145
+        WMNEJ["mnej"] += VABIJ["efmn"] * T1["fj"];                 // :146
+
+        WAMIJ["amij"] = VAIJK["amij"];                             // :148
+        WAMIJ["amij"] += VABCI["efam"] * Tau["efij"];              // :149
+        WAMIJ["amij"] += VAIBJ["amej"] * T1["ei"];                 // :150
+        WAMIJ["amij"] += VAIJB["amie"] * T1["ej"];                 // :151
+        WAMIJ["amij"] -= WMNIJ["nmij"] * T1["an"];                 // :152
+        WAMIJ["amij"] += WMNEJ["nmej"] * T2SA["aein"];             // :153
+        WAMIJ["amij"] -= WMNEJ["mnej"] * T2["aein"];               // :154
+        WAMIJ["amij"] -= WMNEJ["mnei"] * T2["aenj"];               // :155
+        WAMIJ["amij"] += FME["me"] * T2["aeij"];                   // :156
+
+        WAMIE["amie"] = VAIJBSA["amie"];                           // :158
+        WAMIE["amie"] += VABIJSA["efmn"] * T2SA["afin"];           // :159
+        WAMIE["amie"] += 2 * VABCI["feam"] * T1["fi"];             // :160
+        WAMIE["amie"] -= VABCI["efam"] * T1["fi"];                 // :161
+
+        WAMEI["amei"] = VAIBJ["amei"];                             // :163
+        WAMEI["amei"] -= VABIJ["efnm"] * T2["afni"];               // :164
+        WAMEI["amei"] += VABCI["efam"] * T1["fi"];                 // :165
+
+        if (with_wabej) {                                          // :167-181
+            if (!WABEJ.handle()) WABEJ = Tensor(ctx, std::vector<int64_t>{v, v, v, o}, true);
+            WABEJ["abej"] = VABCI["abej"];
+            WABEJ["abej"] += WMNEJ["mnej"] * Tau["abmn"];
+            WABEJ["abej"] -= 0.5 * WAMIE["bmje"] * T1["am"];
+            WABEJ["abej"] -= 0.5 * WAMEI["bmej"] * T1["am"];
+            WABEJ["abej"] -= WAMEI["amej"] * T1["bm"];
+            WABEJ["abej"] += VABCD["abef"] * T1["fj"];
+            WABEJ["abej"] += VABCI["efam"] * T2SA["fbmj"];
+            WABEJ["abej"] -= VABCI["feam"] * T2["fbmj"];
+            WABEJ["abej"] -= VABCI["febm"] * T2["afmj"];
+            WABEJ["abej"] -= FME["me"] * T2["abmj"];
+        }
+
+        WAMIE["amie"] -= 2 * WMNEJ["mnei"] * T1["an"];             // :183
+        WAMIE["amie"] += WMNEJ["nmei"] * T1["an"];                 // :184
+
+        WAMEI["amei"] -= WMNEJ["nmei"] * T1["an"];                 // :186
+    }
 };
 
 extern "C" {
@@ -488,6 +559,14 @@ int aq_rhfccsd_run(aq_rhfccsd* cc, aq_iter_callback cb, void* user, double* ener
     });
 }
 
+int aq_rhfccsd_hbar(aq_rhfccsd* cc, int with_wabej) {
+    return guarded([&] {
+        AQ_CHECK(cc, AQ_ERR_INVALID, "null solver");
+        cc->hbar(with_wabej != 0);
+        AQ_HIP(hipStreamSynchronize(cc->ctx->stream));
+    });
+}
+
 int aq_rhfccsd_tensor(const aq_rhfccsd* cc, int which, const aq_tensor** out) {
     return guarded([&] {
         AQ_CHECK(cc && out, AQ_ERR_INVALID, "null argument");
@@ -506,8 +585,10 @@ int aq_rhfccsd_tensor(const aq_rhfccsd* cc, int which, const aq_tensor** out) {
             case 15: t = &cc->WAMIJ; break;
             case 16: t = &cc->WAMEI; break;
             case 17: t = &cc->WAMIE; break;
+            case 18: t = &cc->WABEJ; break;
             default: throw Error(AQ_ERR_INVALID, "unknown tensor selector");
         }
+        AQ_CHECK(t->handle(), AQ_ERR_INVALID, "WABEJ is formed by aq_rhfccsd_hbar(cc, 1)");
         *out = t->handle();
     });
 }

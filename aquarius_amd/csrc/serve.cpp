@@ -8,7 +8,9 @@
 // except that once the oldest queued job has waited longer than
 // max_bypass_s no later job may overtake it.  Jobs run to convergence with
 // their integrals, amplitudes and DIIS history resident on the device; their
-// kernels overlap on the GPU through the per-slot streams.  Nothing mutable is
+// kernels overlap on the GPU through the per-slot streams.  The slot contexts
+// autotune their GEMMs: the first job of each (o, v) size meets new GEMM shape
+// buckets and times the kernel variants once; later jobs reuse the choices.  Nothing mutable is
 // shared between jobs, so each job's arithmetic is independent of the mix.
 #include "internal.hpp"
 
@@ -195,6 +197,7 @@ int aq_server_create(int device, size_t hbm_budget, int slots, double max_bypass
                 aq_ctx* c = nullptr;
                 tensor::check(aq_ctx_create(device, 0, &c));
                 s->ctxs.push_back(c);
+                tensor::check(aq_ctx_set_autotune(c, 1));
             }
             size_t free_b = 0, total_b = 0;
             AQ_HIP(hipMemGetInfo(&free_b, &total_b));

@@ -80,6 +80,22 @@ int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
  * SIMD (BF16x3 has variant 0 only).  A tuning knob: results are identical up
  * to floating-point summation order. */
 int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
+/* GEMM autotuning (no reference counterpart).  With enable != 0 the first
+ * GEMM of every shape bucket (arithmetic, operand layouts, and M, N, K and
+ * batch in half-octaves) times each kernel variant once on this context's
+ * stream, into a scratch output, and the fastest serves that bucket from then
+ * on for every context on the device.  Variants compute identical results
+ * (the split-K partition does not depend on them), so tuning changes speed
+ * only.  aq_ctx_set_gemm_variant (variant >= 0) overrides the tuner. */
+int aq_ctx_set_autotune(aq_ctx* ctx, int enable);
+typedef struct aq_tune_entry {
+    int device, precision, trans_a, trans_b;
+    int m_bucket, n_bucket, k_bucket, batch_bucket;   /* round(2*log2(x)) */
+    int variant;                                      /* chosen */
+    float ms[3];                                      /* trial time per variant, -1 = not tried */
+} aq_tune_entry;
+/* Copy up to `cap` entries of the tuning table; *n receives the total. */
+int aq_autotune_table(aq_tune_entry* out, int cap, int* n);
 /* Bytes currently held (tensors + caches + pooled scratch) and the budget. */
 int aq_ctx_memory(const aq_ctx* ctx, size_t* in_use, size_t* budget);
 /* High-water mark of the bytes held since creation or the last reset. */
@@ -224,8 +240,17 @@ int aq_rhfccsd_iterate(aq_rhfccsd* cc, double* energy, double* conv);
 typedef void (*aq_iter_callback)(int iter, double energy, double conv, double seconds, int precision, void* user);
 int aq_rhfccsd_run(aq_rhfccsd* cc, aq_iter_callback log_cb, void* user,
                    double* energy, double* conv, int* iterations, int* converged);
+/* Hbar products of the current amplitudes, formed after convergence
+ * (rhfccsd.cxx:108-188): overwrites FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ,
+ * WAMEI, WAMIE with the similarity-transformed Hamiltonian elements the EOM
+ * consumers read (rhfeomeeccsd.cxx:55-80); with_wabej != 0 also forms
+ * WABEJ [v,v,v,o] (the reference forms it when a consumer uses it).  FP64;
+ * synchronising.  A later aq_rhfccsd_iterate overwrites the products again
+ * with the iteration's intermediates. */
+int aq_rhfccsd_hbar(aq_rhfccsd* cc, int with_wabej);
 /* Amplitudes / state: which = 0: T1[v,o], 1: T2[v,v,o,o], 2: Z1, 3: Z2,
- * 4: Tau; 10..17: FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE. */
+ * 4: Tau; 10..17: FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE;
+ * 18: WABEJ (after aq_rhfccsd_hbar(cc, 1)). */
 int aq_rhfccsd_tensor(const aq_rhfccsd* cc, int which, const aq_tensor** out);
 /* Precision the next iteration's contractions will use. */
 int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision);
@@ -322,8 +347,10 @@ int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* C, co
  * waited longer than `max_bypass_s`, in which case admission pauses until it
  * fits.  Admitted jobs run concurrently, their kernels overlapping on the
  * device; each job keeps its integrals, amplitudes and DIIS history resident
- * until it converges.  Jobs share no mutable state, so a job's result does
- * not depend on what else runs. */
+ * until it converges.  The slot contexts autotune their GEMMs
+ * (aq_ctx_set_autotune), sharing one tuning table.  Jobs share no mutable
+ * state that affects arithmetic, so a job's result does not depend on what
+ * else runs. */
 typedef struct aq_server aq_server;
 
 typedef struct aq_job_spec {

@@ -6,6 +6,10 @@ Follows, statement by statement, the reference's
   * task spec / defaults     src/cc/rhfccsd.cxx:332-353
   * DIIS::extrapolate        src/convergence/diis.hpp:16-227
   * Iterative::run           src/util/iterative.hpp:126-188
+  * Hbar products            src/cc/rhfccsd.cxx:108-188 (RHFCCSD.hbar; pinned by
+                             the T = 0 limit, FME = dE/dT1 / 2 and the dependence
+                             on T1 of the first-order terms -- the remaining
+                             T-dependent terms of W are "parity unpinned")
   * Denominator              src/operator/denominator.hpp:21-89
 with the operator semantics of oracle.ctf_ops (CTFTensor, ctf_tensor.cxx:403-514).
 
@@ -293,6 +297,86 @@ class RHFCCSD:
         self.energy = 2 * ops.dot(fAI, "ai", T1, "ai") + ops.dot(VABIJSA, "abij", Tau, "abij")  # :322
         self.conv = max(ops.norm(Z1, 0), ops.norm(Z2, 0))                 # :323
         self.diis.extrapolate([T1, T2], [Z1, Z2])                         # :325-326
+
+    # rhfccsd.cxx:108-188
+    def hbar(self, with_wabej=False):
+        """Hbar products of the current amplitudes, in place (FAE ... WAMIE;
+        WABEJ [v,v,v,o] when asked)."""
+        V = self.V
+        fAI, fAB, fIJ = self.fAI, self.fAB, self.fIJ
+        VABCD, VABCI, VABIJ = V["VABCD"], V["VABCI"], V["VABIJ"]
+        VAIBJ, VAIJB, VAIJK, VIJKL = V["VAIBJ"], V["VAIJB"], V["VAIJK"], V["VIJKL"]
+        T1, T2, Tau, T2SA = self.T1, self.T2, self.Tau, self.T2SA
+        VABIJSA, VAIJBSA = self.VABIJSA, self.VAIJBSA
+        FAE, FMI, FME = self.FAE, self.FMI, self.FME
+        WMNIJ, WMNEJ, WAMIJ = self.WMNIJ, self.WMNEJ, self.WAMIJ
+        WAMEI, WAMIE = self.WAMEI, self.WAMIE
+
+        tsum(1.0, T2, "abij", 0.0, Tau, "abij")                           # :119
+        mult(1.0, T1, "ai", T1, "bj", 1.0, Tau, "abij")                    # :120
+        tsum(2.0, T2, "abij", 0.0, T2SA, "abij")                          # :122
+        tsum(-1.0, T2, "abji", 1.0, T2SA, "abij")                         # :123
+
+        tsum(1.0, fAI, "em", 0.0, FME, "me")                              # :125
+        mult(1.0, VABIJSA, "efmn", T1, "fn", 1.0, FME, "me")              # :126
+
+        tsum(1.0, fIJ, "mi", 0.0, FMI, "mi")                              # :128
+        mult(1.0, VABIJSA, "efmn", T2, "efin", 1.0, FMI, "mi")            # :129
+        mult(1.0, FME, "me", T1, "ei", 1.0, FMI, "mi")                    # :130
+        mult(2.0, VAIJK, "finm", T1, "fn", 1.0, FMI, "mi")                # :131
+        mult(-1.0, VAIJK, "fimn", T1, "fn", 1.0, FMI, "mi")               # :132
+
+        tsum(1.0, fAB, "ae", 0.0, FAE, "ae")                              # :134
+        mult(-1.0, VABIJSA, "efmn", T2, "afmn", 1.0, FAE, "ae")           # :135
+        mult(-1.0, FME, "me", T1, "am", 1.0, FAE, "ae")                   # :136
+        mult(2.0, VABCI, "efam", T1, "fm", 1.0, FAE, "ae")                # :137
+        mult(-1.0, VABCI, "feam", T1, "fm", 1.0, FAE, "ae")               # :138
+
+        tsum(0.5, VIJKL, "mnij", 0.0, WMNIJ, "mnij")                      # :140
+        mult(0.5, VABIJ, "efmn", Tau, "efij", 1.0, WMNIJ, "mnij")         # :141
+        mult(1.0, VAIJK, "ejmn", T1, "ei", 1.0, WMNIJ, "mnij")            # :142
+        tsum(1.0, WMNIJ, "nmji", 1.0, WMNIJ, "mnij")                      # :143 (aliased)
+
+        tsum(1.0, VAIJK, "ejmn", 0.0, WMNEJ, "mnej")                      # :145
+        mult(1.0, VABIJ, "efmn", T1, "fj", 1.0, WMNEJ, "mnej")            # :146
+
+        tsum(1.0, VAIJK, "amij", 0.0, WAMIJ, "amij")                      # :148
+        mult(1.0, VABCI, "efam", Tau, "efij", 1.0, WAMIJ, "amij")         # :149
+        mult(1.0, VAIBJ, "amej", T1, "ei", 1.0, WAMIJ, "amij")            # :150
+        mult(1.0, VAIJB, "amie", T1, "ej", 1.0, WAMIJ, "amij")            # :151
+        mult(-1.0, WMNIJ, "nmij", T1, "an", 1.0, WAMIJ, "amij")           # :152
+        mult(1.0, WMNEJ, "nmej", T2SA, "aein", 1.0, WAMIJ, "amij")        # :153
+        mult(-1.0, WMNEJ, "mnej", T2, "aein", 1.0, WAMIJ, "amij")         # :154
+        mult(-1.0, WMNEJ, "mnei", T2, "aenj", 1.0, WAMIJ, "amij")         # :155
+        mult(1.0, FME, "me", T2, "aeij", 1.0, WAMIJ, "amij")              # :156
+
+        tsum(1.0, VAIJBSA, "amie", 0.0, WAMIE, "amie")                    # :158
+        mult(1.0, VABIJSA, "efmn", T2SA, "afin", 1.0, WAMIE, "amie")      # :159
+        mult(2.0, VABCI, "feam", T1, "fi", 1.0, WAMIE, "amie")            # :160
+        mult(-1.0, VABCI, "efam", T1, "fi", 1.0, WAMIE, "amie")           # :161
+
+        tsum(1.0, VAIBJ, "amei", 0.0, WAMEI, "amei")                      # :163
+        mult(-1.0, VABIJ, "efnm", T2, "afni", 1.0, WAMEI, "amei")         # :164
+        mult(1.0, VABCI, "efam", T1, "fi", 1.0, WAMEI, "amei")            # :165
+
+        if with_wabej:                                                    # :167-181
+            nv, no = self.nv, self.no
+            W = self.WABEJ = np.zeros((nv, nv, nv, no))
+            tsum(1.0, VABCI, "abej", 0.0, W, "abej")
+            mult(1.0, WMNEJ, "mnej", Tau, "abmn", 1.0, W, "abej")
+            mult(-0.5, WAMIE, "bmje", T1, "am", 1.0, W, "abej")
+            mult(-0.5, WAMEI, "bmej", T1, "am", 1.0, W, "abej")
+            mult(-1.0, WAMEI, "amej", T1, "bm", 1.0, W, "abej")
+            mult(1.0, VABCD, "abef", T1, "fj", 1.0, W, "abej")
+            mult(1.0, VABCI, "efam", T2SA, "fbmj", 1.0, W, "abej")
+            mult(-1.0, VABCI, "feam", T2, "fbmj", 1.0, W, "abej")
+            mult(-1.0, VABCI, "febm", T2, "afmj", 1.0, W, "abej")
+            mult(-1.0, FME, "me", T2, "abmj", 1.0, W, "abej")
+
+        mult(-2.0, WMNEJ, "mnei", T1, "an", 1.0, WAMIE, "amie")           # :183
+        mult(1.0, WMNEJ, "nmei", T1, "an", 1.0, WAMIE, "amie")            # :184
+
+        mult(-1.0, WMNEJ, "nmei", T1, "an", 1.0, WAMEI, "amei")           # :186
 
     def converged(self):
         """Iterative::isConverged: every conv strictly below tol (iterative.hpp:178-188)."""

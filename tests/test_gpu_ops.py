@@ -116,6 +116,56 @@ def test_contract_reduced_precision(ctx, prec, tol, ia, ib, ic, o, v):
     check_contract(ctx, ia, ib, ic, dims_for(o, v), beta=0.0, seed=21, tol=tol, prec=prec)
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("ia,ib,ic,dim", [
+    ("ak", "kb", "ab", {"a": 300, "b": 270, "k": 200}),
+    ("amie", "ebmj", "abij", {"a": 40, "b": 40, "e": 40, "m": 8, "i": 8, "j": 8}),
+])
+def test_bf16x3_tile_variants(ctx, variant, ia, ib, ic, dim):
+    """Every BF16x3 block shape (128x128, 256x128, 128x256) on ragged edges."""
+    print(f"case: bf16x3 variant {variant} {ia},{ib}->{ic} {dim}", flush=True)
+    ctx.set_gemm_variant("bf16x3", variant)
+    try:
+        check_contract(ctx, ia, ib, ic, dim, beta=0.5, seed=9, tol=5e-5, prec="bf16x3")
+    finally:
+        ctx.set_gemm_variant("bf16x3", -1)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_gemm_variants_and_autotuner_are_bitwise_identical(ctx, prec):
+    """Kernel variants (and hence the autotuner's choice) never change a
+    result: the split-K partition and the per-element summation order do not
+    depend on the variant.  Shapes: a 128x128-tiled GEMM and a split-K one."""
+    print(f"case: variants/autotune {prec}", flush=True)
+    rng = np.random.default_rng(3)
+    for dim in ({"a": 2048, "b": 2048, "k": 96}, {"a": 300, "b": 260, "k": 4096}):
+        A = ctx.from_numpy(rnd(rng, (dim["a"], dim["k"])))
+        B = ctx.from_numpy(rnd(rng, (dim["k"], dim["b"])))
+        outs = []
+        ctx.set_precision(prec, 0.0)
+        try:
+            for v in (0, 1, 2, -1):
+                ctx.set_gemm_variant(prec, v)
+                ctx.set_autotune(v == -1)
+                Cc = ctx.tensor((dim["a"], dim["b"]), zero=True)
+                ctx.contract(1.0, A, "ak", B, "kb", 0.0, Cc, "ab")
+                outs.append(Cc.numpy())
+                Cc.free()
+        finally:
+            ctx.set_precision("fp64")
+            ctx.set_gemm_variant(prec, -1)
+            ctx.set_autotune(False)
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+        A.free()
+        B.free()
+    table = aq.autotune_table()
+    code = aq.PRECISIONS[prec]
+    mine = [e for e in table if e["precision"] == code and e["m_bucket"] == 22 and e["n_bucket"] == 22]
+    assert mine and mine[0]["variant"] in (0, 1, 2)
+    assert mine[0]["ms"][mine[0]["variant"]] == min(m for m in mine[0]["ms"] if m >= 0)
+
+
 def test_bf16x3_is_more_accurate_than_bf16_alone(ctx):
     """Error of the split product stays near 2^-17 per term, far below the
     2^-9 of single BF16 rounding."""

@@ -241,3 +241,65 @@ def test_iterative_stop_rule_and_nonconvergence():
     assert cc2.run()
     assert cc2.history[-1][1] < cfg2["convergence"]
     assert all(h[1] >= cfg2["convergence"] for h in cc2.history[:-1])
+
+
+# ------------------------------------------------------------ Hbar products
+
+HBAR = ("FAE", "FMI", "FME", "WMNIJ", "WMNEJ", "WAMIJ", "WAMEI", "WAMIE", "WABEJ")
+
+
+def test_hbar_reduces_to_bare_integrals_at_t_zero():
+    """rhfccsd.cxx:108-188 with T1 = T2 = 0: every product is its bare block."""
+    f, V, _, _ = small_system(4, 3, 7, offdiag=0.05)
+    cc = rhfccsd.RHFCCSD(f, V)
+    cc.T1[...] = 0.0
+    cc.T2[...] = 0.0
+    cc.hbar(with_wabej=True)
+    np.testing.assert_array_equal(cc.FAE, f["fAB"])
+    np.testing.assert_array_equal(cc.FMI, f["fIJ"])
+    np.testing.assert_array_equal(cc.FME, f["fAI"].T)
+    np.testing.assert_allclose(cc.WMNIJ, V["VIJKL"], atol=1e-15)          # (<mn|ij> + <nm|ji>)/2
+    np.testing.assert_array_equal(cc.WMNEJ, V["VAIJK"].transpose(2, 3, 0, 1))
+    np.testing.assert_array_equal(cc.WAMIJ, V["VAIJK"])
+    np.testing.assert_allclose(cc.WAMIE, 2 * V["VAIJB"] - V["VAIBJ"].transpose(0, 1, 3, 2), atol=1e-15)
+    np.testing.assert_array_equal(cc.WAMEI, V["VAIBJ"])
+    np.testing.assert_array_equal(cc.WABEJ, V["VABCI"])
+
+
+def test_hbar_fme_is_half_the_energy_gradient():
+    """E = 2 f.T1 + VABIJSA.(T2 + T1 T1) is quadratic in T1 and
+    dE/dT1[e,m] = 2 FME[m,e] exactly; central differences check it."""
+    f, V, _, _ = small_system(5, 3, 6, offdiag=0.05)
+    cc = rhfccsd.RHFCCSD(f, V)
+    rng = np.random.default_rng(1)
+    cc.T1[...] = rng.standard_normal(cc.T1.shape) * 0.05
+    cc.T2[...] = rng.standard_normal(cc.T2.shape) * 0.02
+    cc.hbar()
+
+    def energy(T1):
+        tau = cc.T2 + np.einsum("ai,bj->abij", T1, T1)
+        return 2 * np.sum(f["fAI"] * T1) + np.sum(cc.VABIJSA * tau)
+
+    h = 1e-3
+    grad = np.zeros_like(cc.T1)
+    for e in range(cc.nv):
+        for m in range(cc.no):
+            d = np.zeros_like(cc.T1)
+            d[e, m] = h
+            grad[e, m] = (energy(cc.T1 + d) - energy(cc.T1 - d)) / (2 * h)
+    np.testing.assert_allclose(grad, 2 * cc.FME.T, rtol=0, atol=1e-11)
+
+
+def test_hbar_shapes_and_wabej_order():
+    """WABEJ reads WAMIE/WAMEI before their final T1 terms (rhfccsd.cxx:167-186):
+    forming it must not change the other products."""
+    f, V, _, _ = small_system(6, 3, 5, offdiag=0.05)
+    a = rhfccsd.RHFCCSD(f, V)
+    a.run(max_iterations=5)
+    b = rhfccsd.RHFCCSD(f, V)
+    b.run(max_iterations=5)
+    a.hbar(False)
+    b.hbar(True)
+    for k in HBAR[:-1]:
+        np.testing.assert_array_equal(getattr(a, k), getattr(b, k))
+    assert b.WABEJ.shape == (5, 5, 5, 3)
