@@ -146,6 +146,65 @@ def serving(a):
     print(json.dumps(out))
 
 
+def copy_calibration(n=1 << 27, reps=3):
+    """Plain copies B = A of a 2^27-element FP64 tensor (1 GiB read + 1 GiB
+    written per launch): the known-size reference for the HBM counters."""
+    import aquarius_amd as aq
+    ctx = aq.Context(0)
+    A = ctx.tensor((n,), zero=False)
+    ctx.fill(A, 1.0, 0.0)
+    B = ctx.tensor((n,), zero=True)
+    ctx.enable_timing(True)
+    ctx.timing_reset()
+    for _ in range(reps):
+        ctx.sum(1.0, A, "i", 0.0, B, "i")
+    ctx.sync()
+    tim = ctx.timing()
+    A.free()
+    B.free()
+    ctx.close()
+    print(json.dumps({"copy_calibration": {"elements": n, "bytes_read": 8 * n, "bytes_written": 8 * n,
+                                           "launches": reps, "timing": tim}}))
+
+
+def only(a):
+    """W warmup + exactly K timed iterations of one solve (FP64, or the mixed
+    policy from its first iteration) and nothing else: the command profiled
+    for the per-kernel statistics and HBM counters."""
+    import aquarius_amd as aq
+    o, v = a.o, a.v
+    nq = a.nq or 3 * (o + v)
+    ctx = aq.Context(0)
+    ctx.set_autotune(bool(a.autotune))
+    ints = aq.Integrals.synthetic(ctx, a.seed, o, v, nq)
+    ctx.sync()
+    prec = "fp64" if a.only == "fp64" else a.mixed
+    cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=1000, precision=prec,
+                    refine_below=a.refine_below, min_reduced_flops=1e8)
+    for _ in range(a.warmup):
+        cc.iterate()
+    ctx.enable_timing(True)
+    ctx.timing_reset()
+    ctx.sync()
+    steps = max(a.steps, 1)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        e, c = cc.iterate()
+        log(f"{a.only} it {k + 1} E={e:.12f} conv={c:.2e}")
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    tim = ctx.timing()
+    peak = PEAK["fp64"] if a.only == "fp64" else PEAK[a.mixed]
+    rf = roofline_of(tim, "gemm_f64" if a.only == "fp64" else ("gemm_bf16x3" if a.mixed == "bf16x3" else "gemm_f32"),
+                     peak)
+    cc.close()
+    ints.free()
+    ctx.close()
+    print(json.dumps({"profiling_run": a.only, "o": o, "v": v, "steps": steps, "warmup": a.warmup,
+                      "ms_per_iter": 1e3 * dt / steps, "iter_per_s": steps / dt, "roofline": rf,
+                      "timing": {k: t for k, t in tim.items() if t["launches"] > 0}}))
+
+
 KNOWN_ANSWERS = {   # test/input.aq:20-23 (H2O cc-pVDZ, bohr geometry)
     "h2o-bohr": {"scf": -74.550126456692, "mp2": -0.171348679568, "ccsd": -0.180145524753},
 }
@@ -167,6 +226,7 @@ def molecular(a):
         atoms, units = water_cluster(a.seed, nw), "angstrom"
         label = f"(H2O){nw} cc-pVDZ, seed {a.seed}, 7 A cube, min O-O 2.7 A"
     ctx = aq.Context(0)
+    ctx.set_autotune(bool(a.autotune))
     t = {}
     t0 = time.perf_counter()
     mol = aq.Molecule(atoms, a.basis, units=units)
@@ -249,6 +309,11 @@ def main():
     ap.add_argument("--refine-below", type=float, default=1e-5)
     ap.add_argument("--cpu-sample", default="16,112,2", help="o,v,iterations of the CPU oracle sample ('' to skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solve")
+    ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
+                    help="profiling run: W warmup + K timed iterations of one solve, then exit")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="profiling run: plain device copies of a 1 GiB FP64 tensor (HBM counter calibration)")
+    ap.add_argument("--autotune", type=int, default=1, help="GEMM autotuner on (1) or off (0)")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -263,6 +328,10 @@ def main():
     if a.serving:
         serving(a)
         return
+    if a.copy_calibration:
+        return copy_calibration()
+    if a.only:
+        return only(a)
 
     import numpy as np
     import aquarius_amd as aq
@@ -271,6 +340,7 @@ def main():
     nq = a.nq or 3 * (o + v)
     F = flops_per_iter(o, v)
     ctx = aq.Context(0)
+    ctx.set_autotune(bool(a.autotune))
     t0 = time.perf_counter()
     ints = aq.Integrals.synthetic(ctx, a.seed, o, v, nq)
     ctx.sync()
@@ -353,7 +423,8 @@ def main():
         "config": {"workload": f"synthetic closed-shell RHF-CCSD o={o} v={v} nQ={nq} seed={a.seed}, full vvvv "
                                f"resident; {a.mixed} MFMA contractions until conv<{a.refine_below:g}, then FP64",
                    "o": o, "v": v, "nQ": nq, "seed": a.seed, "sigma": aq.synth_default_sigma(v, nq),
-                   "convergence": a.convergence, "diis_order": a.diis, "flops_per_iter": F},
+                   "convergence": a.convergence, "diis_order": a.diis, "flops_per_iter": F,
+                   "gemm_autotune": bool(a.autotune)},
         "roofline": dominant,
         "mixed": {"iterations_to_converge": n_conv, "energy": e_mixed, "time_to_E_s": t_mixed_to_E,
                   "reduced_iterations_in_timed_window": len(reduced), "phases": phase_rate,
@@ -367,6 +438,11 @@ def main():
         res["speedup_iter_per_s"] = value / out["fp64"]["iter_per_s"]
         res["speedup_time_to_E"] = out["fp64"]["time_to_E_s"] / t_mixed_to_E
     res["serving"] = "separate mode: python bench.py --serving (one JSON line of its own)"
+    tab = aq.autotune_table()
+    res["gemm_autotune"] = {"buckets": len(tab),
+                            "variant_counts": {p: {str(v): sum(1 for e in tab if e["precision"] == c and e["variant"] == v)
+                                                   for v in (0, 1, 2)}
+                                               for p, c in aq.PRECISIONS.items()}}
     ints.free()
     ctx.close()
 

@@ -76,9 +76,10 @@ int aq_ctx_sync(aq_ctx* ctx);
 int aq_ctx_set_precision(aq_ctx* ctx, int precision, double min_reduced_flops);
 int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
 /* Kernel variant of the large-tile GEMM for one arithmetic (AQ_PREC_*):
- * -1 = library default, 0 = BK 16, 1 = BK 32, 2 = BK 16 with two waves per
- * SIMD (BF16x3 has variant 0 only).  A tuning knob: results are identical up
- * to floating-point summation order. */
+ * -1 = library default (or the autotuner's choice), FP64/FP32: 0 = BK 16,
+ * 1 = BK 32, 2 = BK 16 with two waves per SIMD; BF16x3: 0 = 128x128 blocks,
+ * 1 = 256x128, 2 = 128x256.  A tuning knob: every variant computes identical
+ * results. */
 int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
 /* GEMM autotuning (no reference counterpart).  With enable != 0 the first
  * GEMM of every shape bucket (arithmetic, operand layouts, and M, N, K and
