@@ -31,6 +31,13 @@ sys.path.insert(0, ROOT)
 METRIC = "CCSD iter/s mixed vs FP64 at |ΔE|≤1e-6 Eh; served jobs/h + p99 time-to-E"
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 
+# HBM bytes per launch of a kernel class, from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE)
+# of `bench.py --only fp64 --steps 1 --warmup 0`, scaled by the same counters on a plain 1 GiB copy
+# (fetch x1.9999, write x1.0); mean over the class's launches in one iteration.  Keyed by
+# (timing class, o, v); the per-launch table and the commands are in the profile summary
+# gemm_f64_128x128_o48v288_hbm_traffic.txt.
+TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NN", 48, 288): 107.7e9}
+
 
 def flops_per_iter(o, v):
     """Algorithmic FLOPs of one iteration, term by term from rhfccsd.cxx:231-326."""
@@ -415,6 +422,10 @@ def main():
     # dominant kernel of the timed region
     cands = [r for r in (rf_red, rf_64) if r]
     dominant = max(cands, key=lambda r: r["launches"] * r["mean_ms"]) if cands else None
+    for r in (rf_red, rf_64, (out.get("fp64") or {}).get("roofline")):
+        if r:
+            r["traffic"] = TRAFFIC_PER_LAUNCH.get((r["kernel"], o, v))
+            r["traffic_unit"] = "bytes per launch (rocprofv3 --pmc, copy-calibrated)" if r["traffic"] else None
 
     res = {
         "metric": METRIC, "value": value, "unit": "iter/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
