@@ -624,3 +624,62 @@ class SCF:
                   (v, o, o, o), (o, o, o, o)]
         return Integrals(self.ctx, {nm: Tensor(self.ctx, s, handle=_P(out[k]))
                                     for k, (nm, s) in enumerate(zip(FOCK + BLOCKS, shapes))})
+
+
+# ------------------------------------------------------------------ FCIDUMP
+
+_SIGS.update({
+    "aq_mo_rhf_blocks": [_P, _P, C.c_int, _DP, _DP, _DP, C.POINTER(_P)],
+    "aq_fcidump_header": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "aq_fcidump_read": [C.c_char_p, C.c_int, _DP, _DP, _DP],
+    "aq_fcidump_write": [C.c_char_p, C.c_int, C.c_int, C.c_int, _DP, _DP, C.c_double, C.c_double],
+})
+
+
+def fcidump_read(path, full_1e=False):
+    """(h[norb, norb], eri[norb]*4 Mulliken, ecore, norb, nelec, ms2) of an FCIDUMP file."""
+    n, ne, ms2 = C.c_int(), C.c_int(), C.c_int()
+    _ck(lib().aq_fcidump_header(str(path).encode(), C.byref(n), C.byref(ne), C.byref(ms2)))
+    N = n.value
+    h = np.empty(N * N)
+    eri = np.empty(N ** 4)
+    ec = C.c_double()
+    _ck(lib().aq_fcidump_read(str(path).encode(), 1 if full_1e else 0, _dp(h), _dp(eri), C.byref(ec)))
+    return h.reshape((N, N), order="F"), eri.reshape((N,) * 4, order="F"), ec.value, N, ne.value, ms2.value
+
+
+def fcidump_write(path, h, eri, ecore, nelec, ms2=0, threshold=1e-14):
+    h = np.asfortranarray(h, dtype=np.float64)
+    eri = np.asfortranarray(eri, dtype=np.float64)
+    n = h.shape[0]
+    _ck(lib().aq_fcidump_write(str(path).encode(), n, nelec, ms2, h.ctypes.data_as(_DP), eri.ctypes.data_as(_DP),
+                               float(ecore), float(threshold)))
+
+
+def integrals_from_mo(ctx, h, mo_eri, nocc):
+    """CCSD inputs from MO-basis integrals (aq_mo_rhf_blocks): returns
+    (Integrals, electronic SCF energy, f)."""
+    n = h.shape[0]
+    t = ctx.from_numpy(mo_eri)
+    out = (_P * 10)()
+    e = C.c_double()
+    f = np.empty((n, n), order="F")
+    hh = np.asfortranarray(h, dtype=np.float64)
+    try:
+        _ck(lib().aq_mo_rhf_blocks(ctx.h, t.h, nocc, hh.ctypes.data_as(_DP), C.byref(e), f.ctypes.data_as(_DP), out))
+    finally:
+        t.free()
+    o, v = nocc, n - nocc
+    shapes = [(v, o), (v, v), (o, o), (v, v, v, v), (v, v, v, o), (v, v, o, o), (v, o, v, o), (v, o, o, v),
+              (v, o, o, o), (o, o, o, o)]
+    ints = Integrals(ctx, {nm: Tensor(ctx, s, handle=_P(out[k])) for k, (nm, s) in enumerate(zip(FOCK + BLOCKS, shapes))})
+    return ints, e.value, f
+
+
+def integrals_from_fcidump(ctx, path, full_1e=False):
+    """Read an FCIDUMP file and build the CCSD inputs: (Integrals, E_SCF total, norb, nelec)."""
+    h, eri, ecore, n, nelec, ms2 = fcidump_read(path, full_1e)
+    if nelec % 2 or ms2 != 0:
+        raise AqError(4, "the closed-shell engine needs an even electron count and MS2 = 0")
+    ints, e, _ = integrals_from_mo(ctx, h, eri, nelec // 2)
+    return ints, ecore + e, n, nelec

@@ -158,6 +158,58 @@ void eigh(int n, std::vector<double>& A, std::vector<double>& w, std::vector<dou
 }  // namespace fe
 }  // namespace aqamd
 
+namespace {
+
+// The ten CCSD inputs from a full MO tensor mo[p,q,r,s] = (pq|rs) and the MO
+// Fock matrix f (row-major n x n): fAI, fAB, fIJ and the seven Dirac blocks
+// <pq|rs> = (pr|qs), each cut from mo as the box [p-range, r-range,
+// q-range, s-range] and permuted into [p, q, r, s].
+void rhf_blocks(aq_ctx* ctx, const aq_tensor* mo, int nocc, const std::vector<double>& f, aq_tensor** out) {
+    const int64_t n = mo->len[0];
+    const int64_t o = nocc, v = n - nocc;
+    const size_t N = size_t(n);
+    Owned fAI(ctx, {v, o}), fAB(ctx, {v, v}), fIJ(ctx, {o, o});
+    {
+        std::vector<double> h(size_t(v * o));
+        for (int64_t i = 0; i < o; ++i)
+            for (int64_t a = 0; a < v; ++a) h[size_t(a + i * v)] = f[size_t(o + a) * N + size_t(i)];
+        ck(aq_write_all(ctx, fAI.t, h.data()));
+        h.assign(size_t(v * v), 0.0);
+        for (int64_t b = 0; b < v; ++b)
+            for (int64_t a = 0; a < v; ++a) h[size_t(a + b * v)] = f[size_t(o + a) * N + size_t(o + b)];
+        ck(aq_write_all(ctx, fAB.t, h.data()));
+        h.assign(size_t(o * o), 0.0);
+        for (int64_t j = 0; j < o; ++j)
+            for (int64_t i = 0; i < o; ++i) h[size_t(i + j * o)] = f[size_t(i) * N + size_t(j)];
+        ck(aq_write_all(ctx, fIJ.t, h.data()));
+    }
+    auto block = [&](int r0, int r1, int r2, int r3, Owned& dst) {
+        const int64_t lp = r0 == 0 ? o : v, lq = r1 == 0 ? o : v, lr = r2 == 0 ? o : v, ls = r3 == 0 ? o : v;
+        const int64_t off[2] = {0, o};
+        Owned box(ctx, {lp, lr, lq, ls});
+        const int64_t startA[4] = {off[r0 != 0], off[r2 != 0], off[r1 != 0], off[r3 != 0]};
+        const int64_t startB[4] = {0, 0, 0, 0};
+        const int64_t ext[4] = {lp, lr, lq, ls};
+        ck(aq_slice(ctx, 1.0, mo, startA, 0.0, box.t, startB, ext));
+        ck(aq_tensor_create(ctx, 4, std::vector<int64_t>{lp, lq, lr, ls}.data(), 0, &dst.t));
+        ck(aq_sum(ctx, 1.0, box.t, "prqs", 0.0, dst.t, "pqrs"));
+    };
+    const int V_ = 1, O_ = 0;
+    Owned VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL;
+    block(V_, V_, V_, V_, VABCD);
+    block(V_, V_, V_, O_, VABCI);
+    block(V_, V_, O_, O_, VABIJ);
+    block(V_, O_, V_, O_, VAIBJ);
+    block(V_, O_, O_, V_, VAIJB);
+    block(V_, O_, O_, O_, VAIJK);
+    block(O_, O_, O_, O_, VIJKL);
+    ck(aq_ctx_sync(ctx));
+    Owned* all[10] = {&fAI, &fAB, &fIJ, &VABCD, &VABCI, &VABIJ, &VAIBJ, &VAIJB, &VAIJK, &VIJKL};
+    for (int k = 0; k < 10; ++k) out[k] = all[k]->release();
+}
+
+}  // namespace
+
 extern "C" {
 
 int aq_molecule_create(int natoms, const int* Z, const double* xyz, const char* genbas, int spherical, int charge,
@@ -360,7 +412,6 @@ int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* Ccm, 
         const int64_t n = eri->len[0];
         AQ_CHECK(eri->len[1] == n && eri->len[2] == n && eri->len[3] == n, AQ_ERR_INVALID, "AO tensor must be [n,n,n,n]");
         AQ_CHECK(nocc >= 1 && nocc < n, AQ_ERR_INVALID, "need 1 <= nocc < nao");
-        const int64_t o = nocc, v = n - nocc;
         const size_t N = size_t(n);
         // Fock matrix in the MO basis, f = C^T F C
         std::vector<double> CF(N * N, 0.0), f(N * N, 0.0);
@@ -376,21 +427,6 @@ int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* Ccm, 
                 for (size_t l = 0; l < N; ++l) s += CF[p * N + l] * Ccm[l + q * N];
                 f[p * N + q] = s;
             }
-        Owned fAI(ctx, {v, o}), fAB(ctx, {v, v}), fIJ(ctx, {o, o});
-        {
-            std::vector<double> h(size_t(v * o));
-            for (int64_t i = 0; i < o; ++i)
-                for (int64_t a = 0; a < v; ++a) h[size_t(a + i * v)] = f[size_t(o + a) * N + size_t(i)];
-            ck(aq_write_all(ctx, fAI.t, h.data()));
-            h.assign(size_t(v * v), 0.0);
-            for (int64_t b = 0; b < v; ++b)
-                for (int64_t a = 0; a < v; ++a) h[size_t(a + b * v)] = f[size_t(o + a) * N + size_t(o + b)];
-            ck(aq_write_all(ctx, fAB.t, h.data()));
-            h.assign(size_t(o * o), 0.0);
-            for (int64_t j = 0; j < o; ++j)
-                for (int64_t i = 0; i < o; ++i) h[size_t(i + j * o)] = f[size_t(i) * N + size_t(j)];
-            ck(aq_write_all(ctx, fIJ.t, h.data()));
-        }
         // four quarter transformations
         Owned Cd(ctx, {n, n});
         ck(aq_write_all(ctx, Cd.t, Ccm));
@@ -403,32 +439,39 @@ int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* Ccm, 
             ck(aq_contract(ctx, 1.0, Cd.t, "st", x1.t, "pqrs", 0.0, mo.t, "pqrt"));
         }
         ck(aq_ctx_trim(ctx));
-        // <pq|rs> = (pr|qs): cut the box [p-range, r-range, q-range, s-range]
-        // out of the MO tensor, then permute it into [p, q, r, s].
-        auto block = [&](int64_t r0, int64_t r1, int64_t r2, int64_t r3, Owned& dst) {
-            // dst index order p q r s; source box order p r q s
-            const int64_t lp = r0 == 0 ? o : v, lq = r1 == 0 ? o : v, lr = r2 == 0 ? o : v, ls = r3 == 0 ? o : v;
-            const int64_t off[2] = {0, o};
-            Owned box(ctx, {lp, lr, lq, ls});
-            const int64_t startA[4] = {off[r0 != 0], off[r2 != 0], off[r1 != 0], off[r3 != 0]};
-            const int64_t startB[4] = {0, 0, 0, 0};
-            const int64_t ext[4] = {lp, lr, lq, ls};
-            ck(aq_slice(ctx, 1.0, mo.t, startA, 0.0, box.t, startB, ext));
-            ck(aq_tensor_create(ctx, 4, std::vector<int64_t>{lp, lq, lr, ls}.data(), 0, &dst.t));
-            ck(aq_sum(ctx, 1.0, box.t, "prqs", 0.0, dst.t, "pqrs"));
-        };
-        const int V_ = 1, O_ = 0;
-        Owned VABCD, VABCI, VABIJ, VAIBJ, VAIJB, VAIJK, VIJKL;
-        block(V_, V_, V_, V_, VABCD);
-        block(V_, V_, V_, O_, VABCI);
-        block(V_, V_, O_, O_, VABIJ);
-        block(V_, O_, V_, O_, VAIBJ);
-        block(V_, O_, O_, V_, VAIJB);
-        block(V_, O_, O_, O_, VAIJK);
-        block(O_, O_, O_, O_, VIJKL);
-        ck(aq_ctx_sync(ctx));
-        Owned* all[10] = {&fAI, &fAB, &fIJ, &VABCD, &VABCI, &VABIJ, &VAIBJ, &VAIJB, &VAIJK, &VIJKL};
-        for (int k = 0; k < 10; ++k) out[k] = all[k]->release();
+        rhf_blocks(ctx, mo.t, nocc, f, out);
+    });
+}
+
+int aq_mo_rhf_blocks(aq_ctx* ctx, const aq_tensor* mo_eri, int nocc, const double* h, double* escf_elec,
+                     double* f_out, aq_tensor** out) {
+    return guarded([&] {
+        AQ_CHECK(ctx && mo_eri && h && out, AQ_ERR_INVALID, "null argument");
+        AQ_CHECK(mo_eri->ndim == 4, AQ_ERR_INVALID, "MO tensor must be 4-dimensional");
+        const int64_t n = mo_eri->len[0];
+        AQ_CHECK(mo_eri->len[1] == n && mo_eri->len[2] == n && mo_eri->len[3] == n, AQ_ERR_INVALID,
+                 "MO tensor must be [n,n,n,n]");
+        AQ_CHECK(nocc >= 1 && nocc < n, AQ_ERR_INVALID, "need 1 <= nocc < norb");
+        const size_t N = size_t(n);
+        // closed-shell Fock of the orbitals as given: f = h + 2 J[D] - K[D], D = occupied projector
+        std::vector<double> D(N * N, 0.0), J(N * N), K(N * N), f(N * N);
+        for (int k = 0; k < nocc; ++k) D[size_t(k) * N + size_t(k)] = 1.0;
+        Owned Dd(ctx, {n, n}), Jd(ctx, {n, n}), Kd(ctx, {n, n});
+        ck(aq_write_all(ctx, Dd.t, D.data()));
+        ck(aq_contract(ctx, 1.0, mo_eri, "pqkl", Dd.t, "kl", 0.0, Jd.t, "pq"));
+        ck(aq_contract(ctx, 1.0, mo_eri, "pklq", Dd.t, "kl", 0.0, Kd.t, "pq"));
+        ck(aq_read_all(ctx, Jd.t, J.data()));
+        ck(aq_read_all(ctx, Kd.t, K.data()));
+        double e = 0.0;
+        // h, J and K in column-major key order (p + q n); f row-major for the block cutter
+        for (size_t p = 0; p < N; ++p)
+            for (size_t q = 0; q < N; ++q) f[p * N + q] = h[p + q * N] + 2.0 * J[p + q * N] - K[p + q * N];
+        for (int i = 0; i < nocc; ++i) e += h[size_t(i) * (N + 1)] + f[size_t(i) * (N + 1)];
+        if (escf_elec) *escf_elec = e;
+        if (f_out)
+            for (size_t p = 0; p < N; ++p)
+                for (size_t q = 0; q < N; ++q) f_out[p + q * N] = f[p * N + q];
+        rhf_blocks(ctx, mo_eri, nocc, f, out);
     });
 }
 

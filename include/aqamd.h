@@ -338,6 +338,32 @@ int aq_scf_rhf(aq_ctx* ctx, const aq_molecule* mol, const aq_tensor* eri, const 
 int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* C, const double* F,
                 aq_tensor** out);
 
+/* CCSD inputs from integrals already in an orthonormal MO basis (the role the
+ * reference's FCIDUMP task plays, src/operator/fcidump.cxx:80-517): mo_eri is
+ * a device tensor [n,n,n,n] with mo[p,q,r,s] = (pq|rs), h[p + q*n] the
+ * one-electron integrals, the first nocc orbitals doubly occupied.  Builds the
+ * closed-shell Fock matrix f = h + 2 J - K of those orbitals (J, K as device
+ * contractions), returns the electronic SCF energy sum_i (h_ii + f_ii) (add
+ * the core energy for the total) and, optionally, f[p + q*n], and creates the
+ * ten tensors in the order of aq_synth_build. */
+int aq_mo_rhf_blocks(aq_ctx* ctx, const aq_tensor* mo_eri, int nocc, const double* h, double* escf_elec,
+                     double* f, aq_tensor** out);
+
+/* FCIDUMP files (Knowles-Handy format), read as the reference's FCIDUMP task
+ * reads them (src/operator/fcidump.cxx:43-140): header up to "/", "&END" or
+ * "$END" with NORB, NELEC (and MS2); lines "value p q r s" with p = 0 the
+ * core energy, q = 0 ignored, r = 0 the one-electron h_pq (mirrored to h_qp
+ * unless full_1e), else (pq|rs) filling all eight symmetric positions (a line
+ * whose ordered pair (p,q) precedes (r,s) is skipped, as there).  Outputs
+ * h[p + q*norb], eri[p + q*n + r*n^2 + s*n^3] (norb^4 doubles) and the core
+ * energy.  The writer emits the canonical order (i>=j, k>=l, ij>=kl) of the
+ * integrals whose magnitude is at least `threshold`, then h (i>=j), then the
+ * core energy. */
+int aq_fcidump_header(const char* path, int* norb, int* nelec, int* ms2);
+int aq_fcidump_read(const char* path, int full_1e, double* h, double* eri, double* ecore);
+int aq_fcidump_write(const char* path, int norb, int nelec, int ms2, const double* h, const double* eri,
+                     double ecore, double threshold);
+
 /* ---- serving: continuous batching of CCSD jobs (no reference counterpart;
  * Aquarius runs one task DAG sequentially, src/task/task.cxx:495-588) -------
  *

@@ -80,3 +80,27 @@ def test_h2o_angstrom_matches_oracle(ctx):
     assert rm["converged"] and abs(rm["energy"] - r64["energy"]) < 1e-9
     mixed.close()
     ints.free()
+
+
+def test_fcidump_to_ccsd_known_answers(ctx, tmp_path):
+    """An FCIDUMP of the H2O/cc-pVDZ SCF orbitals (written from the oracle's
+    integrals) read by the library, the closed-shell Fock and blocks built on
+    the device, then CCSD: the reference's SCF and CCSD known answers."""
+    print("case: FCIDUMP H2O cc-pVDZ bohr -> device blocks -> CCSD", flush=True)
+    b = I.molecule(H2O_REFERENCE_BOHR, aq.basis_text("cc-pVDZ"))
+    S, T, V = I.one_electron(b)
+    eri = I.two_electron(b)
+    E, eps, C, F = I.rhf(S, T + V, eri, 5, b.nuclear_repulsion())
+    h = C.T @ (T + V) @ C
+    mo = np.einsum("abcd,ap,bq,cr,ds->pqrs", eri, C, C, C, C, optimize=True)
+    path = tmp_path / "FCIDUMP"
+    aq.fcidump_write(path, h, mo, b.nuclear_repulsion(), nelec=10)
+    ints, escf, n, nelec = aq.integrals_from_fcidump(ctx, path)
+    assert (n, nelec) == (24, 10)
+    assert abs(escf - (-74.550126456692)) < 1e-9                        # test/input.aq:20
+    cc = aq.RHFCCSD(ctx, ints, convergence=1e-11, max_iterations=100)
+    assert abs(cc.mp2 - (-0.171348679568)) < 1e-9                       # test/input.aq:21
+    r = cc.run()
+    assert abs(r["energy"] - (-0.180145524753)) < 1e-9                  # test/input.aq:23
+    cc.close()
+    ints.free()
