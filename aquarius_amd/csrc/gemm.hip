@@ -244,8 +244,11 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
 
 // ------------------------------------------------------------------ BF16x3
 
-template <int BM, int BN, int WM, int WN>
-__global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
+// PF = 1: the next k-tile's global loads are issued before this tile's
+// MFMAs.  PF = 2: two register sets, the tile two steps ahead is loaded, so a
+// load has two tiles of MFMA work (~1500 cycles) to arrive instead of one.
+template <int BM, int BN, int WM, int WN, int PF>
+__global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g) {
     constexpr int BK = 32;
     constexpr int KP = BK + 8;                      // padded row (bf16 elements)
     constexpr int WTM = BM / WM, WTN = BN / WN;
@@ -253,6 +256,7 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
     constexpr int VA = BM * BK / 8, VB = BN * BK / 8;   // 8-element vectors per plane
     constexpr int NA = (VA + 255) / 256, NB = (VB + 255) / 256;
     static_assert(WM * WN == 4, "4 waves");
+    static_assert(PF == 1 || PF == 2, "prefetch depth");
 
     __shared__ i16x8 dummy_align;                   // keeps LDS 16-byte aligned
     __shared__ __attribute__((aligned(16))) unsigned short As[2][2][BM][KP];   // [buf][hi/lo][m][k]
@@ -276,51 +280,51 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
     const unsigned short* A = static_cast<const unsigned short*>(g.A) + int64_t(b) * g.strideA;
     const unsigned short* B = static_cast<const unsigned short*>(g.B) + int64_t(b) * g.strideB;
 
-    i16x8 ra[2][NA], rb[2][NB];
-    auto load = [&](int64_t k0) {
+    // loads without branches: an element outside the operand reads the
+    // operand's first vector and is replaced by zeros
+    auto load = [&](int64_t k0, i16x8 (&xa)[2][NA], i16x8 (&xb)[2][NB]) {
+        const i16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
             const int e = tid + 256 * r;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
             const int64_t m = m0 + mm, k = k0 + kk;
-            i16x8 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
-            if (e < VA && m < g.M && k < kend) {
-                h = *reinterpret_cast<const i16x8*>(A + k + m * g.lda);
-                l = *reinterpret_cast<const i16x8*>(A + g.planeA + k + m * g.lda);
-            }
-            ra[0][r] = h;
-            ra[1][r] = l;
+            const bool ok = e < VA && m < g.M && k < kend;
+            const unsigned short* p = ok ? A + k + m * g.lda : A;
+            const i16x8 h = *reinterpret_cast<const i16x8*>(p);
+            const i16x8 l = *reinterpret_cast<const i16x8*>(p + g.planeA);
+            xa[0][r] = ok ? h : z;
+            xa[1][r] = ok ? l : z;
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
             const int e = tid + 256 * r;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
             const int64_t n = n0 + nn, k = k0 + kk;
-            i16x8 h = {0, 0, 0, 0, 0, 0, 0, 0}, l = h;
-            if (e < VB && n < g.N && k < kend) {
-                h = *reinterpret_cast<const i16x8*>(B + k + n * g.ldb);
-                l = *reinterpret_cast<const i16x8*>(B + g.planeB + k + n * g.ldb);
-            }
-            rb[0][r] = h;
-            rb[1][r] = l;
+            const bool ok = e < VB && n < g.N && k < kend;
+            const unsigned short* p = ok ? B + k + n * g.ldb : B;
+            const i16x8 h = *reinterpret_cast<const i16x8*>(p);
+            const i16x8 l = *reinterpret_cast<const i16x8*>(p + g.planeB);
+            xb[0][r] = ok ? h : z;
+            xb[1][r] = ok ? l : z;
         }
     };
-    auto store = [&](int buf) {
+    auto store = [&](int buf, const i16x8 (&xa)[2][NA], const i16x8 (&xb)[2][NB]) {
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
             const int e = tid + 256 * r;
             if (e >= VA) continue;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
-            *reinterpret_cast<i16x8*>(&As[buf][0][mm][kk]) = ra[0][r];
-            *reinterpret_cast<i16x8*>(&As[buf][1][mm][kk]) = ra[1][r];
+            *reinterpret_cast<i16x8*>(&As[buf][0][mm][kk]) = xa[0][r];
+            *reinterpret_cast<i16x8*>(&As[buf][1][mm][kk]) = xa[1][r];
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
             const int e = tid + 256 * r;
             if (e >= VB) continue;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
-            *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][kk]) = rb[0][r];
-            *reinterpret_cast<i16x8*>(&Bs[buf][1][nn][kk]) = rb[1][r];
+            *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][kk]) = xb[0][r];
+            *reinterpret_cast<i16x8*>(&Bs[buf][1][nn][kk]) = xb[1][r];
         }
     };
 
@@ -330,32 +334,22 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    if (nk > 0) load(kbeg);
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int buf = int(kt & 1);
-        store(buf);
-        __syncthreads();
-        if (kt + 1 < nk) load(kbeg + (kt + 1) * BK);
-        // src0 (16 x 32): lane holds row lane%16, k = 8*(lane/16) .. +7  (from the B tile)
-        // src1 (32 x 16): lane holds col lane%16, k = 8*(lane/16) .. +7  (from the A tile)
+    // src0 (16 x 32): lane holds row lane%16, k = 8*(lane/16) .. +7  (from the B tile)
+    // src1 (32 x 16): lane holds col lane%16, k = 8*(lane/16) .. +7  (from the A tile)
+    auto compute = [&](int buf) {
         const int kq = 8 * (lane / 16);
         bf16x8 xh[FN], xl[FN], yh[FM], yl[FM];
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int nn = wn * WTN + j * 16 + lane % 16;
-            i16x8 h = *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][kq]);
-            i16x8 l = *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][kq]);
-            xh[j] = __builtin_bit_cast(bf16x8, h);
-            xl[j] = __builtin_bit_cast(bf16x8, l);
+            xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][kq]));
+            xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][kq]));
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
             const int mm = wm * WTM + i * 16 + lane % 16;
-            i16x8 h = *reinterpret_cast<const i16x8*>(&As[buf][0][mm][kq]);
-            i16x8 l = *reinterpret_cast<const i16x8*>(&As[buf][1][mm][kq]);
-            yh[i] = __builtin_bit_cast(bf16x8, h);
-            yl[i] = __builtin_bit_cast(bf16x8, l);
+            yh[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][kq]));
+            yl[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][kq]));
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -365,6 +359,37 @@ __global__ __launch_bounds__(256) void k_gemm_bf16x3(GemmArgs g) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i], acc[i][j], 0, 0, 0);
             }
+    };
+
+    const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    i16x8 ra0[2][NA], rb0[2][NB];
+    if (PF == 1) {
+        if (nk > 0) load(kbeg, ra0, rb0);
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int buf = int(kt & 1);
+            store(buf, ra0, rb0);
+            __syncthreads();
+            if (kt + 1 < nk) load(kbeg + (kt + 1) * BK, ra0, rb0);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(buf);
+        }
+    } else {
+        i16x8 ra1[2][NA], rb1[2][NB];
+        if (nk > 0) load(kbeg, ra0, rb0);
+        if (nk > 1) load(kbeg + BK, ra1, rb1);
+        for (int64_t kt = 0; kt < nk; kt += 2) {
+            store(0, ra0, rb0);
+            __syncthreads();
+            if (kt + 2 < nk) load(kbeg + (kt + 2) * BK, ra0, rb0);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(0);
+            if (kt + 1 >= nk) break;
+            store(1, ra1, rb1);
+            __syncthreads();
+            if (kt + 3 < nk) load(kbeg + (kt + 3) * BK, ra1, rb1);
+            __builtin_amdgcn_sched_barrier(0);
+            compute(1);
+        }
     }
 
     double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
@@ -418,9 +443,9 @@ static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid, int variant) {
 // single wave per SIMD exposes (FP64 35 -> 56 TF/s; FP32 unchanged at ~108).
 int default_gemm_variant(Fmt fmt) { return fmt == FMT_BF16X2 ? 0 : 2; }
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, int PF = 1>
 static void run_cfg_bf16(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
-    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN>), grid, dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN, PF>), grid, dim3(256), 0, ctx->stream, a);
 }
 
 // Kernel launch for one (format, block shape, variant); the caller checked the shape.
@@ -436,7 +461,8 @@ static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const Ge
         else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
         else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
     } else {
-        if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
+        if (BM == 128 && BN == 128 && variant == 3) run_cfg_bf16<128, 128, 2, 2, 2>(ctx, a, grid);
+        else if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
         else if (BM == 256) run_cfg_bf16<256, 128, 2, 2>(ctx, a, grid);
         else if (BN == 256) run_cfg_bf16<128, 256, 2, 2>(ctx, a, grid);
         else if (BM == 64) run_cfg_bf16<64, 64, 2, 2>(ctx, a, grid);
@@ -475,7 +501,7 @@ struct TuneKey {
 
 struct TuneEntry {
     int variant = -1;
-    float ms[3] = {-1.f, -1.f, -1.f};
+    float ms[4] = {-1.f, -1.f, -1.f, -1.f};
 };
 
 std::mutex g_tune_mu;
@@ -514,10 +540,11 @@ static int tuned_variant(aq_ctx* ctx, const GemmDesc& d, int BM, int BN, const G
         AQ_HIP(hipEventCreate(&e0));
         AQ_HIP(hipEventCreate(&e1));
         float best = 0.f;
-        for (int v = 0; v < 3; ++v) {
+        const int nvar = d.fmt == FMT_BF16X2 ? 4 : 3;
+        for (int v = 0; v < nvar; ++v) {
             int bm = BM, bn = BN;
             variant_shape(d.fmt, v, d.M, d.N, bm, bn);
-            if (d.fmt == FMT_BF16X2 && v > 0 && bm == BM && bn == BN) continue;   // same kernel as variant 0
+            if (d.fmt == FMT_BF16X2 && (v == 1 || v == 2) && bm == BM && bn == BN) continue;   // = variant 0
             const int64_t tiles = ((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn);
             dim3 grid(unsigned(tiles), 1, unsigned(a.batch * a.nsplit));
             dispatch(ctx, d.fmt, bm, bn, v, t, grid);           // warm
@@ -629,7 +656,7 @@ int autotune_entries(aq_tune_entry* out, int cap) {
             e.k_bucket = kv.first.k;
             e.batch_bucket = kv.first.batch;
             e.variant = kv.second.variant;
-            for (int v = 0; v < 3; ++v) e.ms[v] = kv.second.ms[v];
+            for (int v = 0; v < 4; ++v) e.ms[v] = kv.second.ms[v];
         }
         ++n;
     }
