@@ -521,7 +521,7 @@ class SCFConfig(C.Structure):
 
 class SCFResult(C.Structure):
     _fields_ = [("energy", C.c_double), ("conv", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
-                ("nocc", C.c_int)]
+                ("nocc", C.c_int), ("nocc_beta", C.c_int)]
 
 
 _SIGS.update({
@@ -683,3 +683,104 @@ def integrals_from_fcidump(ctx, path, full_1e=False):
         raise AqError(4, "the closed-shell engine needs an even electron count and MS2 = 0")
     ints, e, _ = integrals_from_mo(ctx, h, eri, nelec // 2)
     return ints, ecore + e, n, nelec
+
+
+# ------------------------------------------------- open shells (spin-orbital)
+
+SO_INPUTS = ("fAI", "fME", "fAE", "fMI", "VABIJ", "VMNEF", "VAMEF", "VABEJ", "VABEF", "VMNIJ", "VMNEJ", "VAMIJ",
+             "VAMEI")
+
+_SIGS.update({
+    "aq_scf_uhf": [_P, _P, _P, C.c_int, C.POINTER(SCFConfig), C.POINTER(SCFResult), _DP, _DP, _DP],
+    "aq_aomo_uhf": [_P, _P, C.c_int, C.c_int, _DP, _DP, C.POINTER(_P)],
+    "aq_soccsd_create": [_P, C.POINTER(_P), C.POINTER(RHFCCSDConfig), C.c_int, C.POINTER(_P)],
+    "aq_soccsd_destroy": [_P],
+    "aq_soccsd_mp2": [_P, _DP],
+    "aq_soccsd_iterate": [_P, _DP, _DP],
+    "aq_soccsd_run": [_P, ITER_CB, C.c_void_p, _DP, _DP, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "aq_soccsd_tensor": [_P, C.c_int, C.POINTER(_P)],
+})
+
+
+class UHF:
+    """Unrestricted SCF on the device (localaoscf, uhf.cxx) and the
+    spin-orbital integrals of the spin-orbital CCSD path."""
+
+    def __init__(self, ctx, mol, multiplicity=1, eri=None, convergence=None, max_iterations=None, diis_order=None,
+                 nthreads=0):
+        self.ctx, self.mol = ctx, mol
+        self.eri = eri if eri is not None else mol.ints_2e_device(ctx, nthreads)
+        cfg = SCFConfig()
+        _ck(lib().aq_scf_default_config(C.byref(cfg)))
+        for k, val in (("convergence", convergence), ("max_iterations", max_iterations), ("diis_order", diis_order)):
+            if val is not None:
+                setattr(cfg, k, val)
+        n = mol.nao
+        self.eps = np.empty(2 * n)
+        self.C = np.empty(2 * n * n)
+        self.F = np.empty(2 * n * n)
+        r = SCFResult()
+        _ck(lib().aq_scf_uhf(ctx.h, mol.h, self.eri.h, multiplicity, C.byref(cfg), C.byref(r), _dp(self.eps),
+                             _dp(self.C), _dp(self.F)))
+        self.energy, self.conv, self.iterations = r.energy, r.conv, r.iterations
+        self.converged, self.nalpha, self.nbeta = bool(r.converged), r.nocc, r.nocc_beta
+
+    def spin_orbital_integrals(self):
+        out = (_P * 13)()
+        _ck(lib().aq_aomo_uhf(self.ctx.h, self.eri.h, self.nalpha, self.nbeta, _dp(self.C), _dp(self.F), out))
+        n = self.mol.nao
+        o = self.nalpha + self.nbeta
+        v = 2 * n - o
+        shapes = [(v, o), (o, v), (v, v), (o, o), (v, v, o, o), (o, o, v, v), (v, o, v, v), (v, v, v, o),
+                  (v, v, v, v), (o, o, o, o), (o, o, v, o), (v, o, o, o), (v, o, v, o)]
+        return Integrals(self.ctx, {nm: Tensor(self.ctx, s, handle=_P(out[k]))
+                                    for k, (nm, s) in enumerate(zip(SO_INPUTS, shapes))})
+
+
+class SpinOrbitalCCSD:
+    """aq_soccsd: spin-orbital CCSD (CCSD<U>, ccsd.cxx) on UHF integrals."""
+
+    def __init__(self, ctx, ints, config=None, ccd=False, **over):
+        self.ctx = ctx
+        cfg = config if config is not None else default_config(**over)
+        ins = (_P * 13)(*[ints.t[k].h for k in SO_INPUTS])
+        self.h = _P()
+        _ck(lib().aq_soccsd_create(ctx.h, ins, C.byref(cfg), 1 if ccd else 0, C.byref(self.h)))
+        self.history = []
+
+    @property
+    def mp2(self):
+        r = C.c_double()
+        _ck(lib().aq_soccsd_mp2(self.h, C.byref(r)))
+        return r.value
+
+    def iterate(self):
+        e, c = C.c_double(), C.c_double()
+        _ck(lib().aq_soccsd_iterate(self.h, C.byref(e), C.byref(c)))
+        return e.value, c.value
+
+    def run(self):
+        hist = self.history
+
+        def _cb(it, e, c, dt, prec, user):
+            hist.append({"iter": it, "energy": e, "conv": c, "seconds": dt, "precision": prec})
+
+        cb = ITER_CB(_cb)
+        e, c, n, ok = C.c_double(), C.c_double(), C.c_int(), C.c_int()
+        _ck(lib().aq_soccsd_run(self.h, cb, None, C.byref(e), C.byref(c), C.byref(n), C.byref(ok)))
+        return {"energy": e.value, "conv": c.value, "iterations": n.value, "converged": bool(ok.value)}
+
+    def tensor(self, name):
+        which = {"T1": 0, "T2": 1, "Z1": 2, "Z2": 3, "Tau": 4}[name]
+        p = _P()
+        _ck(lib().aq_soccsd_tensor(self.h, which, C.byref(p)))
+        n = C.c_int()
+        _ck(lib().aq_tensor_ndim(p, C.byref(n)))
+        lens = np.zeros(n.value, dtype=np.int64)
+        _ck(lib().aq_tensor_lengths(p, lens.ctypes.data_as(_I64P)))
+        return Tensor(self.ctx, tuple(lens), handle=p, owned=False)
+
+    def close(self):
+        if self.h:
+            _ck(lib().aq_soccsd_destroy(self.h))
+            self.h = _P()

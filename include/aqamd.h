@@ -322,7 +322,8 @@ typedef struct aq_scf_result {
     double conv;        /* max |X^T (FDS - SDF) X| of the last iteration */
     int iterations;
     int converged;
-    int nocc;
+    int nocc;           /* occupied (alpha) orbitals */
+    int nocc_beta;      /* occupied beta orbitals (= nocc for RHF) */
 } aq_scf_result;
 int aq_scf_default_config(aq_scf_config* cfg);
 /* eri: the device AO tensor of aq_ints_2e_device.  Outputs (optional):
@@ -337,6 +338,40 @@ int aq_scf_rhf(aq_ctx* ctx, const aq_molecule* mol, const aq_tensor* eri, const 
  * VIJKL. */
 int aq_aomo_rhf(aq_ctx* ctx, const aq_tensor* eri, int nocc, const double* C, const double* F,
                 aq_tensor** out);
+
+/* ---- open shells: UHF and the spin-orbital CCSD path -------------------- */
+
+/* localaoscf for any multiplicity (UHF, src/scf/uhf.cxx:17-409): n_alpha =
+ * (N + mult - 1)/2, n_beta = (N - mult + 1)/2.  Outputs (optional): eps[2*nao]
+ * alpha then beta orbital energies, C[2*nao*nao] alpha then beta column-major
+ * coefficients, F[2*nao*nao] the final alpha and beta AO Fock matrices. */
+int aq_scf_uhf(aq_ctx* ctx, const aq_molecule* mol, const aq_tensor* eri, int multiplicity,
+               const aq_scf_config* cfg, aq_scf_result* res, double* eps, double* C, double* F);
+/* Spin-orbital integrals of a UHF reference (the spin-orbital aomoints task,
+ * src/operator/aomoints.cxx, as CCSD<U> reads them, src/cc/ccsd.cxx:92-105):
+ * dense spin orbitals ordered occupied alpha, occupied beta, virtual alpha,
+ * virtual beta (o = nalpha + nbeta, v = 2 nao - o).  C and F as written by
+ * aq_scf_uhf.  Creates 13 tensors: fAI [v,o], fME [o,v], fAE [v,v], fMI [o,o],
+ * and the antisymmetrised <pq||rs> blocks VABIJ [v,v,o,o], VMNEF [o,o,v,v],
+ * VAMEF [v,o,v,v], VABEJ [v,v,v,o], VABEF [v,v,v,v], VMNIJ [o,o,o,o],
+ * VMNEJ [o,o,v,o], VAMIJ [v,o,o,o], VAMEI [v,o,v,o]. */
+int aq_aomo_uhf(aq_ctx* ctx, const aq_tensor* eri, int nalpha, int nbeta, const double* C, const double* F,
+                aq_tensor** out);
+/* Spin-orbital CCSD (CCSD<U>, src/cc/ccsd.cxx:30-193) on the 13 tensors of
+ * aq_aomo_uhf, which it borrows.  The products into antisymmetric outputs are
+ * antisymmetrised over index pairs taken from different operands, as the
+ * reference's spin-orbital tensors do implicitly.  cfg as for aq_rhfccsd
+ * (the task's defaults are the same); ccd != 0 holds T1 at zero (the CCD
+ * equations).  aq_soccsd_tensor: 0 T1 [v,o], 1 T2 [v,v,o,o], 2 Z1, 3 Z2, 4 Tau. */
+typedef struct aq_soccsd aq_soccsd;
+int aq_soccsd_create(aq_ctx* ctx, const aq_tensor* const* in, const aq_rhfccsd_config* cfg, int ccd,
+                     aq_soccsd** out);
+int aq_soccsd_destroy(aq_soccsd* cc);
+int aq_soccsd_mp2(const aq_soccsd* cc, double* mp2);
+int aq_soccsd_iterate(aq_soccsd* cc, double* energy, double* conv);
+int aq_soccsd_run(aq_soccsd* cc, aq_iter_callback log_cb, void* user,
+                  double* energy, double* conv, int* iterations, int* converged);
+int aq_soccsd_tensor(const aq_soccsd* cc, int which, const aq_tensor** out);
 
 /* CCSD inputs from integrals already in an orthonormal MO basis (the role the
  * reference's FCIDUMP task plays, src/operator/fcidump.cxx:80-517): mo_eri is

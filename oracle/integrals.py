@@ -486,3 +486,58 @@ def molecule(spec_atoms, genbas_text, units="bohr", spherical=True):
     scale = ANGSTROM_TO_BOHR if units == "angstrom" else 1.0
     atoms = [(atomic_number(s), (x * scale, y * scale, z * scale)) for s, x, y, z in spec_atoms]
     return Basis(atoms, parse_genbas(genbas_text), spherical)
+
+
+def uhf(S, H, eri, nalpha, nbeta, enuc, convergence=1e-12, max_iterations=300, diis_order=8):
+    """Unrestricted SCF (uhf.cxx:17-409): core guess for both spins, Pulay
+    DIIS on the stacked orthonormal-basis commutators, stop when the energy
+    change and the largest commutator element are below `convergence`
+    (x100 for the commutator).
+
+    Returns (E_total, (eps_a, eps_b), (Ca, Cb), (Fa, Fb))."""
+    s, U = np.linalg.eigh(S)
+    X = U @ np.diag(s ** -0.5) @ U.T
+
+    def diag(F):
+        e, Cp = np.linalg.eigh(X.T @ F @ X)
+        return e, X @ Cp
+
+    ea, Ca = diag(H)
+    eb, Cb = ea.copy(), Ca.copy()
+    E_old = 0.0
+    hist_f, hist_e = [], []
+    for it in range(max_iterations):
+        Da = Ca[:, :nalpha] @ Ca[:, :nalpha].T
+        Db = Cb[:, :nbeta] @ Cb[:, :nbeta].T
+        J = np.einsum("mnls,ls->mn", eri, Da + Db)
+        Ka = np.einsum("mlns,ls->mn", eri, Da)
+        Kb = np.einsum("mlns,ls->mn", eri, Db)
+        Fa = H + J - Ka
+        Fb = H + J - Kb
+        E = 0.5 * np.sum(Da * (H + Fa)) + 0.5 * np.sum(Db * (H + Fb)) + enuc
+        err_a = X.T @ (Fa @ Da @ S - S @ Da @ Fa) @ X
+        err_b = X.T @ (Fb @ Db @ S - S @ Db @ Fb) @ X
+        conv = max(np.abs(err_a).max(), np.abs(err_b).max())
+        if it > 0 and abs(E - E_old) < convergence and conv < 100 * convergence:
+            ea, Ca = diag(Fa)
+            eb, Cb = diag(Fb)
+            return E, (ea, eb), (Ca, Cb), (Fa, Fb)
+        E_old = E
+        hist_f.append((Fa, Fb))
+        hist_e.append(np.concatenate([err_a.ravel(), err_b.ravel()]))
+        hist_f, hist_e = hist_f[-diis_order:], hist_e[-diis_order:]
+        m = len(hist_f)
+        if m >= 2:
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(m):
+                    B[i, j] = hist_e[i] @ hist_e[j]
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            c = np.linalg.lstsq(B, rhs, rcond=None)[0][:m]
+            Fa = sum(ci * f[0] for ci, f in zip(c, hist_f))
+            Fb = sum(ci * f[1] for ci, f in zip(c, hist_f))
+        ea, Ca = diag(Fa)
+        eb, Cb = diag(Fb)
+    raise RuntimeError("UHF did not converge")

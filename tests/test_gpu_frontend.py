@@ -104,3 +104,67 @@ def test_fcidump_to_ccsd_known_answers(ctx, tmp_path):
     assert abs(r["energy"] - (-0.180145524753)) < 1e-9                  # test/input.aq:23
     cc.close()
     ints.free()
+
+
+CH2_BOHR = [("C", 0.0, 0.0, 0.11726921), ("H", 0.75698224, 0.0, -0.46907685), ("H", -0.75698224, 0.0, -0.46907685)]
+
+
+def test_ch2_triplet_uhf_spin_orbital_ccsd_known_answers(ctx):
+    """Open shell end to end on the device: UHF (device J/K), spin-orbital
+    integrals, spin-orbital CCSD and CCD (ccsd.cxx): the reference's CH2
+    triplet known answers (test/input.aq:67-70) to 1e-9 Eh, and the oracle's
+    per-iteration CCSD energies (same equations, canonical orbitals) to 1e-10."""
+    print("case: CH2 triplet cc-pVDZ bohr, UHF + spin-orbital CCSD/CCD", flush=True)
+    from oracle import socc
+    mol = aq.Molecule(CH2_BOHR, "cc-pVDZ", units="bohr")
+    uhf = aq.UHF(ctx, mol, multiplicity=3)
+    assert uhf.converged and (uhf.nalpha, uhf.nbeta) == (5, 3)
+    assert abs(uhf.energy - (-37.090409355231)) < 1e-9                 # test/input.aq:67
+    ints = uhf.spin_orbital_integrals()
+    cc = aq.SpinOrbitalCCSD(ctx, ints, convergence=1e-11, max_iterations=100, diis_order=8)
+    assert abs(cc.mp2 - (-0.085465008793)) < 1e-9                      # test/input.aq:68
+    r = cc.run()
+    assert r["converged"]
+    assert abs(r["energy"] - (-0.099500248606)) < 1e-9                 # test/input.aq:70
+    T2 = cc.tensor("T2").numpy()
+    np.testing.assert_allclose(T2, -T2.transpose(1, 0, 2, 3), rtol=0, atol=1e-12)
+    cc.close()
+    cd = aq.SpinOrbitalCCSD(ctx, ints, convergence=1e-11, max_iterations=100, diis_order=8, ccd=True)
+    assert abs(cd.run()["energy"] - (-0.098998464453)) < 1e-9          # test/input.aq:69
+    cd.close()
+    # trajectory against the oracle restatement
+    b = I.molecule(CH2_BOHR, aq.basis_text("cc-pVDZ"))
+    S, T, V = I.one_electron(b)
+    eri = I.two_electron(b)
+    E, eps, Cs, Fs = I.uhf(S, T + V, eri, 5, 3, b.nuclear_repulsion())
+    f, g, _ = socc.spin_orbitals(Cs[0], Cs[1], Fs[0], Fs[1], eri, 5, 3)
+    cfg = R.default_config()
+    cfg["convergence"], cfg["max_iterations"], cfg["diis"]["order"] = 1e-11, 100, 8
+    ref = socc.SpinOrbitalCCSD(f, g, 8, cfg)
+    gpu = aq.SpinOrbitalCCSD(ctx, ints, convergence=1e-11, max_iterations=100, diis_order=8)
+    for it in range(10):
+        ref.iterate()
+        e, c = gpu.iterate()
+        assert abs(e - ref.energy) < 1e-10, (it, e, ref.energy)
+    gpu.close()
+    ints.free()
+
+
+def test_h2o_closed_shell_through_spin_orbital_path(ctx):
+    """A closed shell through UHF + spin-orbital CCSD equals the RHF engine."""
+    print("case: H2O DZ bohr, UHF singlet + spin-orbital CCSD vs RHF engine", flush=True)
+    mol = aq.Molecule(H2O_REFERENCE_BOHR, "DZ", units="bohr")
+    uhf = aq.UHF(ctx, mol, multiplicity=1)
+    assert abs(uhf.energy - (-74.491837303542)) < 1e-9                 # test/input.aq:42
+    so_ints = uhf.spin_orbital_integrals()
+    so = aq.SpinOrbitalCCSD(ctx, so_ints, convergence=1e-11, max_iterations=100, diis_order=8)
+    assert abs(so.mp2 - (-0.089766142431)) < 1e-9                      # test/input.aq:43
+    e_so = so.run()["energy"]
+    so.close()
+    so_ints.free()
+    scf = aq.SCF(ctx, mol, eri=uhf.eri)
+    ints = scf.integrals()
+    rc = aq.RHFCCSD(ctx, ints, convergence=1e-11, max_iterations=100, diis_order=8)
+    assert abs(rc.run()["energy"] - e_so) < 1e-10
+    rc.close()
+    ints.free()
