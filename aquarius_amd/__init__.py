@@ -255,6 +255,12 @@ class Tensor:
             n *= s
         return n
 
+    def copy(self):
+        """Deep copy on the device (aq_tensor_copy)."""
+        h = _P()
+        _ck(lib().aq_tensor_copy(self.ctx.h, self.h, C.byref(h)))
+        return Tensor(self.ctx, self.shape, handle=h)
+
     def free(self):
         if self.h and self.owned:
             _ck(lib().aq_tensor_destroy(self.h))

@@ -287,7 +287,17 @@ def molecular(a):
         hist = cc.history
         red = [h["seconds"] for h in hist if h["precision"] != aq.AQ_PREC_FP64]
         full = [h["seconds"] for h in hist if h["precision"] == aq.AQ_PREC_FP64]
+        if name == "fp64":
+            t2_ref = cc.tensor("T2").copy()
+            dt2 = 0.0
+        else:
+            d = cc.tensor("T2").copy()
+            ctx.sum(-1.0, t2_ref, "abij", 1.0, d, "abij")
+            dt2 = ctx.norm(d, 0)
+            d.free()
+            t2_ref.free()
         runs[name] = {"energy": r["energy"], "mp2": mp2, "iterations": r["iterations"], "converged": r["converged"],
+                      "max_abs_dT2_vs_fp64": dt2,
                       "time_to_E_s": dt, "iter_per_s": len(hist) / sum(h["seconds"] for h in hist),
                       "reduced_iterations": len(red),
                       "reduced_iter_per_s": len(red) / sum(red) if red else None,
@@ -344,6 +354,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling run: plain device copies of a 1 GiB FP64 tensor (HBM counter calibration)")
     ap.add_argument("--autotune", type=int, default=1, help="GEMM autotuner on (1) or off (0)")
+    ap.add_argument("--compare-precisions", type=int, default=1,
+                    help="also solve with the other reduced arithmetic (FP32 MFMA beside BF16x3) and report its "
+                         "energy and T2 error against FP64")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -379,6 +392,18 @@ def main():
 
     warm_tuner(a, ctx, ints)
     out = {}
+    t2_64 = None
+
+    def max_dt2(cc):
+        """max |T2 - T2(FP64)| of a converged solver."""
+        if t2_64 is None:
+            return None
+        d = cc.tensor("T2").copy()
+        ctx.sum(-1.0, t2_64, "abij", 1.0, d, "abij")
+        r = ctx.norm(d, 0)
+        d.free()
+        return r
+
     # ---- FP64 reference solve (to convergence)
     e64 = None
     if not a.no_fp64:
@@ -397,6 +422,7 @@ def main():
                        "time_to_E_s": t64, "iter_per_s": len(steady) / sum(steady),
                        "ms_per_iter_median": 1e3 * float(np.median(steady)),
                        "roofline": roofline_of(tim64, "gemm_f64", PEAK["fp64"])}
+        t2_64 = cc.tensor("T2").copy()
         cc.close()
 
     # ---- mixed solve: W warmup, exactly K timed iterations, then on to convergence
@@ -435,7 +461,29 @@ def main():
             n_conv = len(hist)
     t_mixed_to_E = time.perf_counter() - tstart
     e_mixed = hist[(n_conv or len(hist)) - 1][0]
+    dt2_mixed = max_dt2(cc)
     cc.close()
+
+    # ---- the other reduced arithmetic (FP32 MFMA when the mixed run was
+    # BF16x3): same policy, to convergence, for its energy and T2 error
+    other = None
+    if a.compare_precisions and e64 is not None:
+        prec2 = "fp32" if a.mixed == "bf16x3" else "bf16x3"
+        c2 = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
+                        precision=prec2, refine_below=a.refine_below, min_reduced_flops=1e8)
+        t0 = time.perf_counter()
+        r2 = c2.run(callback=lambda h: log(f"{prec2} it {h['iter']:2d} E={h['energy']:.12f} conv={h['conv']:.2e} "
+                                             f"{h['seconds'] * 1e3:.0f} ms"))
+        t2s = time.perf_counter() - t0
+        h2 = c2.history
+        red = [h["seconds"] for h in h2 if h["precision"] != aq.AQ_PREC_FP64]
+        other = {"precision": prec2, "iterations": r2["iterations"], "converged": r2["converged"],
+                 "time_to_E_s": t2s, "iter_per_s": len(h2) / sum(h["seconds"] for h in h2),
+                 "reduced_iterations": len(red), "reduced_iter_per_s": len(red) / sum(red) if red else None,
+                 "dE_vs_fp64": r2["energy"] - e64, "max_abs_dT2_vs_fp64": max_dt2(c2)}
+        c2.close()
+    if t2_64 is not None:
+        t2_64.free()
 
     value = steps / t_timed
     reduced = [h for h in hist[a.warmup:a.warmup + steps] if h[2] != aq.AQ_PREC_FP64]
@@ -469,6 +517,8 @@ def main():
     }
     if e64 is not None:
         res["dE_mixed_vs_fp64"] = e_mixed - e64
+        res["max_abs_dT2_mixed_vs_fp64"] = dt2_mixed
+        res["other_reduced_precision"] = other
         res["fp64_iter_per_s"] = out["fp64"]["iter_per_s"]
         res["speedup_iter_per_s"] = value / out["fp64"]["iter_per_s"]
         res["speedup_time_to_E"] = out["fp64"]["time_to_E_s"] / t_mixed_to_E
