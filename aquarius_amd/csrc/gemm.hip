@@ -280,8 +280,7 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
     const unsigned short* A = static_cast<const unsigned short*>(g.A) + int64_t(b) * g.strideA;
     const unsigned short* B = static_cast<const unsigned short*>(g.B) + int64_t(b) * g.strideB;
 
-    // loads without branches: an element outside the operand reads the
-    // operand's first vector and is replaced by zeros
+    // predicated loads: elements outside the operand are zeros
     auto load = [&](int64_t k0, i16x8 (&xa)[2][NA], i16x8 (&xb)[2][NB]) {
         const i16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
@@ -289,24 +288,26 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
             const int e = tid + 256 * r;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
             const int64_t m = m0 + mm, k = k0 + kk;
-            const bool ok = e < VA && m < g.M && k < kend;
-            const unsigned short* p = ok ? A + k + m * g.lda : A;
-            const i16x8 h = *reinterpret_cast<const i16x8*>(p);
-            const i16x8 l = *reinterpret_cast<const i16x8*>(p + g.planeA);
-            xa[0][r] = ok ? h : z;
-            xa[1][r] = ok ? l : z;
+            i16x8 h = z, l = z;
+            if (e < VA && m < g.M && k < kend) {
+                h = *reinterpret_cast<const i16x8*>(A + k + m * g.lda);
+                l = *reinterpret_cast<const i16x8*>(A + g.planeA + k + m * g.lda);
+            }
+            xa[0][r] = h;
+            xa[1][r] = l;
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
             const int e = tid + 256 * r;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
             const int64_t n = n0 + nn, k = k0 + kk;
-            const bool ok = e < VB && n < g.N && k < kend;
-            const unsigned short* p = ok ? B + k + n * g.ldb : B;
-            const i16x8 h = *reinterpret_cast<const i16x8*>(p);
-            const i16x8 l = *reinterpret_cast<const i16x8*>(p + g.planeB);
-            xb[0][r] = ok ? h : z;
-            xb[1][r] = ok ? l : z;
+            i16x8 h = z, l = z;
+            if (e < VB && n < g.N && k < kend) {
+                h = *reinterpret_cast<const i16x8*>(B + k + n * g.ldb);
+                l = *reinterpret_cast<const i16x8*>(B + g.planeB + k + n * g.ldb);
+            }
+            xb[0][r] = h;
+            xb[1][r] = l;
         }
     };
     auto store = [&](int buf, const i16x8 (&xa)[2][NA], const i16x8 (&xb)[2][NB]) {
