@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <exception>
 #include <cmath>
 #include <mutex>
 #include <thread>
@@ -430,15 +431,26 @@ void ints_2e(const MoleculeData& m, int nthreads, double* eri) {
     if (nthreads <= 0) nthreads = int(std::max(1u, std::thread::hardware_concurrency()));
     nthreads = std::min(nthreads, 64);
 
+    // a failure in a worker (e.g. a host allocation) stops the others and is
+    // rethrown on the calling thread once every worker has joined
     auto run = [&](auto&& body, size_t n) {
         std::atomic<size_t> next{0};
+        std::exception_ptr err;
+        std::mutex err_mu;
         std::vector<std::thread> pool;
         for (int t = 0; t < nthreads; ++t)
             pool.emplace_back([&] {
-                QuartetWork w;
-                for (size_t i; (i = next.fetch_add(1)) < n;) body(i, w);
+                try {
+                    QuartetWork w;
+                    for (size_t i; (i = next.fetch_add(1)) < n;) body(i, w);
+                } catch (...) {
+                    std::lock_guard<std::mutex> lk(err_mu);
+                    if (!err) err = std::current_exception();
+                    next.store(n);
+                }
             });
         for (auto& th : pool) th.join();
+        if (err) std::rethrow_exception(err);
     };
 
     // shell pairs (a >= b) and their Schwarz factors
