@@ -33,7 +33,7 @@ class RHFCCSDConfig(C.Structure):
     _fields_ = [("convergence", C.c_double), ("max_iterations", C.c_int), ("diis_order", C.c_int),
                 ("diis_start", C.c_int), ("diis_damping", C.c_double), ("precision", C.c_int),
                 ("refine_below", C.c_double), ("min_reduced_flops", C.c_double), ("fused_update", C.c_int),
-                ("diis_reset_on_refine", C.c_int), ("fp32_stage_below", C.c_double)]
+                ("diis_reset_on_refine", C.c_int), ("fp32_stage_below", C.c_double), ("graph", C.c_int)]
 
 
 class TuneEntry(C.Structure):
@@ -95,6 +95,7 @@ _SIGS = {
     "aq_rhfccsd_tensor": [_P, C.c_int, C.POINTER(_P)],
     "aq_rhfccsd_hbar": [_P, C.c_int],
     "aq_rhfccsd_current_precision": [_P, C.POINTER(C.c_int)],
+    "aq_rhfccsd_graph_stats": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "aq_synth_host": [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _DP, _DP],
     "aq_synth_default_sigma": [C.c_int, C.c_int],
     "aq_synth_build": [_P, C.c_int, C.c_int, C.c_int, _DP, _DP, C.POINTER(_P)],
@@ -422,6 +423,12 @@ class RHFCCSD:
         r = C.c_int()
         _ck(lib().aq_rhfccsd_current_precision(self.h, C.byref(r)))
         return r.value
+
+    def graph_stats(self):
+        """(captures, replays, abandoned captures) of the contraction-stage HIP graph."""
+        c, r, f = C.c_int(), C.c_int(), C.c_int()
+        _ck(lib().aq_rhfccsd_graph_stats(self.h, C.byref(c), C.byref(r), C.byref(f)))
+        return c.value, r.value, f.value
 
 
 # ---------------------------------------------------------------- serving

@@ -28,7 +28,10 @@ void* dmalloc(aq_ctx* ctx, size_t bytes) {
 }
 
 void dfree(aq_ctx* ctx, void* p) {
-    if (p) (void)hipFreeAsync(p, ctx->stream);
+    if (p) {
+        (void)hipFreeAsync(p, ctx->stream);
+        ctx->mem_gen += 1;
+    }
 }
 
 // Return the memory the device's default pool holds but no allocation uses
@@ -70,7 +73,14 @@ void evict_cache(aq_ctx* ctx, size_t need) {
     }
 }
 
+void invalidate_cache(aq_ctx* ctx) {
+    for (auto& kv : ctx->cache) kv.second.valid = false;
+}
+
 static void* raw_alloc(aq_ctx* ctx, size_t bytes) {
+    // under graph capture an allocation would become part of the graph;
+    // the capturing caller abandons the capture and runs eagerly instead
+    AQ_CHECK(!ctx->capture, AQ_ERR_UNSUPPORTED, "device allocation during graph capture");
     if (ctx_in_use(ctx) + bytes > ctx->budget) {
         pool_release_all(ctx);
         evict_cache(ctx, bytes);

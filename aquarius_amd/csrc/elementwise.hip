@@ -214,6 +214,21 @@ void launch_fill(aq_ctx* ctx, double* x, int64_t n, double alpha, double beta) {
     AQ_HIP(hipGetLastError());
 }
 
+// Zero `bytes` (a multiple of 4) with a kernel: the clear stays an ordinary
+// kernel node when the stream is being captured into a graph.
+__global__ __launch_bounds__(TPB) void k_zero_u32(uint32_t* x, int64_t n) {
+    for (int64_t i = int64_t(blockIdx.x) * TPB + threadIdx.x; i < n; i += int64_t(gridDim.x) * TPB) x[i] = 0u;
+}
+
+void launch_zero_bytes(aq_ctx* ctx, void* p, size_t bytes) {
+    AQ_CHECK(bytes % 4 == 0, AQ_ERR_INVALID, "zero fill needs a multiple of 4 bytes");
+    const int64_t n = int64_t(bytes / 4);
+    if (n <= 0) return;
+    TimedLaunch tl(ctx, "zero", 0.0, double(bytes));
+    hipLaunchKernelGGL(k_zero_u32, dim3(grid_for(n, TPB * 4)), dim3(TPB), 0, ctx->stream, static_cast<uint32_t*>(p), n);
+    AQ_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------- div / invert
 
 // C = beta*C + alpha*A/B (A null: alpha/B); untouched where |B| <= DBL_MIN

@@ -85,6 +85,17 @@ struct CacheEntry {
     std::list<std::string>::iterator lru;
 };
 
+// Recorded while a context's stream is being captured into a HIP graph: the
+// cached operand copies the captured work reads without having rewritten them
+// earlier in the same capture, with the source tensor's version at capture
+// time.  A replay is valid only while those versions are unchanged (the graph
+// reads such a copy as it stands).  Copies the graph rewrites itself are
+// recomputed from the current data by every replay.
+struct CaptureLog {
+    std::vector<std::pair<const aq_tensor*, uint64_t>> fresh_reads;
+    std::vector<const void*> rewritten;
+};
+
 }  // namespace aqamd
 
 struct aq_tensor {
@@ -122,6 +133,13 @@ struct aq_ctx {
     double* d_partials = nullptr;   // device: 8 slots x 1024 partials, then 64 result slots
     double* h_result = nullptr;     // pinned host result slots (64)
 
+    // HIP-graph capture: mem_gen counts frees of device memory (a captured
+    // graph holds raw pointers and is valid only while nothing it uses was
+    // freed); `capture` is non-null while the stream is being captured, and
+    // then nothing may allocate
+    uint64_t mem_gen = 0;
+    aqamd::CaptureLog* capture = nullptr;
+
     // timing
     bool timing = false;
     std::vector<aqamd::PendingTiming> pending;
@@ -141,6 +159,7 @@ double* tensor_alloc(aq_ctx* ctx, size_t bytes);     // persistent (counted agai
 void tensor_free(aq_ctx* ctx, double* p, size_t bytes);
 size_t ctx_in_use(const aq_ctx* ctx);
 void evict_cache(aq_ctx* ctx, size_t need);
+void invalidate_cache(aq_ctx* ctx);                  // every cached copy must be rebuilt before use
 
 struct Scratch {  // RAII pooled device buffer
     aq_ctx* ctx;
@@ -182,6 +201,7 @@ struct StridedPlan {
 
 // ---- kernels (elementwise.hip)
 void launch_fill(aq_ctx* ctx, double* x, int64_t n, double alpha, double beta);
+void launch_zero_bytes(aq_ctx* ctx, void* p, size_t bytes);   // bytes % 4 == 0; a kernel, not a memset
 void launch_strided_sum(aq_ctx* ctx, const StridedPlan& p, double alpha, const double* src,
                         double beta, double* dst);
 void launch_strided_convert(aq_ctx* ctx, const StridedPlan& p, const double* src, void* dst, Fmt fmt,

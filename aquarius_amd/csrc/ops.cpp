@@ -231,7 +231,7 @@ static const void* materialize(aq_ctx* ctx, const Labeled& src, const std::strin
     dst.stride = stride;
     StridedPlan p = make_plan(src, dst);
     auto clear = [&](void* ptr) {
-        if (padded) AQ_HIP(hipMemsetAsync(ptr, 0, bytes, ctx->stream));
+        if (padded) launch_zero_bytes(ctx, ptr, bytes);
     };
     const bool cacheable = src.t != nullptr && !src.has_repeat;
     if (cacheable) {
@@ -246,9 +246,15 @@ static const void* materialize(aq_ctx* ctx, const Labeled& src, const std::strin
                 launch_strided_convert(ctx, p, src.data, e.ptr, fmt, n);
                 e.version = src.t->version;
                 e.valid = true;
+                if (ctx->capture) ctx->capture->rewritten.push_back(e.ptr);
+            } else if (ctx->capture) {
+                const auto& rw = ctx->capture->rewritten;
+                if (std::find(rw.begin(), rw.end(), e.ptr) == rw.end())
+                    ctx->capture->fresh_reads.emplace_back(src.t, src.t->version);
             }
             return e.ptr;
         }
+        AQ_CHECK(!ctx->capture, AQ_ERR_UNSUPPORTED, "operand-cache miss during graph capture");
         if (it != ctx->cache.end()) {   // size changed (cannot happen for a live tensor id)
             ctx->cache_lru.erase(it->second.lru);
             dfree(ctx, it->second.ptr);

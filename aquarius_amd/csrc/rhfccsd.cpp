@@ -117,14 +117,9 @@ struct aq_rhfccsd {
         diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));   // :98
     }
 
-    // rhfccsd.cxx:194-327
-    void iterate() {
-        const int prec = next_precision();
-        if (cfg.diis_reset_on_refine && iter > 0 && prec == AQ_PREC_FP64 && last_prec != AQ_PREC_FP64)
-            diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));
-        last_prec = prec;
-        tensor::check(aq_ctx_set_precision(ctx, prec, cfg.min_reduced_flops));
-
+    // The contraction stage of one iteration (rhfccsd.cxx:231-309): every
+    // operation is stream-ordered and none synchronises with the host.
+    void contractions() {
         Tau["abij"] = T2["abij"];                                  // :231
         Tau["abij"] += T1["ai"] * T1["bj"];                        // :232
         TauSA["abij"] = 2 * Tau["abij"];                           // :234
@@ -192,6 +187,103 @@ struct aq_rhfccsd {
         Z2["abij"] -= WAMEI["amej"] * T2["ebim"];                  // :307  ring
 
         Z2["abij"] += Z2["baji"];                                  // :309 (aliased)
+    }
+
+    // HIP graph of the contraction stage.  Captured when the previous stage
+    // ran eagerly at the same precision (so every scratch buffer is in the
+    // pool and every operand copy exists), replayed while the precision, the
+    // context's memory generation and the versions of the copies it reads
+    // without refreshing are unchanged.  Refreshing copies of amplitudes and
+    // intermediates are kernels inside the graph, so a replay recomputes them
+    // from the current data, exactly as the eager stage would.
+    struct StageGraph {
+        hipGraphExec_t exec = nullptr;
+        int prec = -1;
+        uint64_t gen = 0;
+        std::vector<std::pair<const aq_tensor*, uint64_t>> fresh;
+        int warm_prec = -1;     // precision of the last eager stage
+        int captures = 0, replays = 0, failed = 0;
+    } graph;
+
+    void drop_graph() {
+        if (!graph.exec) return;
+        (void)hipStreamSynchronize(ctx->stream);   // the last replay may still run
+        (void)hipGraphExecDestroy(graph.exec);
+        graph.exec = nullptr;
+        graph.fresh.clear();
+        graph.warm_prec = -1;   // warm the pool and the copies eagerly before the next capture
+    }
+
+    bool graph_valid(int prec) const {
+        if (!graph.exec || graph.prec != prec || graph.gen != ctx->mem_gen) return false;
+        for (const auto& f : graph.fresh)
+            if (f.first->version != f.second) return false;
+        return true;
+    }
+
+    // Capture the stage and launch the graph; false (nothing executed, every
+    // operand copy marked stale) when the capture had to allocate or failed.
+    bool capture_stage(int prec) {
+        CaptureLog log;
+        const uint64_t gen0 = ctx->mem_gen;
+        AQ_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        ctx->capture = &log;
+        bool ok = true;
+        try {
+            contractions();
+        } catch (...) {
+            ok = false;
+        }
+        ctx->capture = nullptr;
+        hipGraph_t g = nullptr;
+        if (hipStreamEndCapture(ctx->stream, &g) != hipSuccess || !g) ok = false;
+        if (ok && ctx->mem_gen != gen0) ok = false;
+        for (const auto& f : log.fresh_reads)   // written after it was read unrefreshed
+            if (f.first->version != f.second) ok = false;
+        hipGraphExec_t exec = nullptr;
+        if (ok && hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) != hipSuccess) ok = false;
+        if (g) (void)hipGraphDestroy(g);
+        if (!ok) {
+            (void)hipGetLastError();
+            if (exec) (void)hipGraphExecDestroy(exec);
+            // the capture marked copies fresh whose conversions never ran
+            invalidate_cache(ctx);
+            graph.failed += 1;
+            return false;
+        }
+        AQ_HIP(hipGraphLaunch(exec, ctx->stream));
+        graph.exec = exec;
+        graph.prec = prec;
+        graph.gen = ctx->mem_gen;
+        graph.fresh = std::move(log.fresh_reads);
+        graph.captures += 1;
+        return true;
+    }
+
+    void stage(int prec) {
+        const bool want = cfg.graph != 0 && !ctx->timing && graph.failed < 2;
+        if (graph.exec && !(want && graph_valid(prec))) drop_graph();
+        if (graph.exec) {
+            AQ_HIP(hipGraphLaunch(graph.exec, ctx->stream));
+            graph.replays += 1;
+            return;
+        }
+        if (want && graph.warm_prec == prec && capture_stage(prec)) return;
+        contractions();
+        graph.warm_prec = prec;
+    }
+
+    ~aq_rhfccsd() { drop_graph(); }
+
+    // rhfccsd.cxx:194-327
+    void iterate() {
+        const int prec = next_precision();
+        if (cfg.diis_reset_on_refine && iter > 0 && prec == AQ_PREC_FP64 && last_prec != AQ_PREC_FP64)
+            diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));
+        last_prec = prec;
+        tensor::check(aq_ctx_set_precision(ctx, prec, cfg.min_reduced_flops));
+
+        stage(prec);                                               // :231-309
 
         // the residual, energy and convergence are FP64 whatever the contraction precision
         tensor::check(aq_ctx_set_precision(ctx, AQ_PREC_FP64, 0.0));
@@ -305,6 +397,7 @@ int aq_rhfccsd_default_config(aq_rhfccsd_config* c) {
         c->fused_update = 1;
         c->diis_reset_on_refine = 0;
         c->fp32_stage_below = 0.0;
+        c->graph = 0;
     });
 }
 
@@ -434,6 +527,15 @@ int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision) {
     return guarded([&] {
         AQ_CHECK(cc && precision, AQ_ERR_INVALID, "null argument");
         *precision = cc->next_precision();
+    });
+}
+
+int aq_rhfccsd_graph_stats(const aq_rhfccsd* cc, int* captures, int* replays, int* failed) {
+    return guarded([&] {
+        AQ_CHECK(cc, AQ_ERR_INVALID, "null solver");
+        if (captures) *captures = cc->graph.captures;
+        if (replays) *replays = cc->graph.replays;
+        if (failed) *failed = cc->graph.failed;
     });
 }
 

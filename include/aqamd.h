@@ -217,6 +217,16 @@ typedef struct aq_rhfccsd_config {
      * fp32_stage_below <= conv < refine_below, and FP64 only below it
      * (0 = switch straight to FP64 at refine_below) */
     double fp32_stage_below;
+    /* 1: run each iteration's contraction stage (rhfccsd.cxx:231-309) as a
+     * HIP graph: captured on the second iteration at a contraction precision
+     * (scratch pool and operand copies warm) and replayed on later ones;
+     * dropped and captured again after a precision switch or whenever the
+     * context frees device memory.  A capture that would allocate is
+     * abandoned and that iteration runs eagerly (two such failures turn
+     * graphs off for the solver).  Same kernels and arguments as the eager
+     * path, so the results are bitwise identical.  Ignored while the context
+     * records kernel timings.  0: every iteration eager. */
+    int graph;
 } aq_rhfccsd_config;
 
 int aq_rhfccsd_default_config(aq_rhfccsd_config* cfg);
@@ -255,6 +265,10 @@ int aq_rhfccsd_hbar(aq_rhfccsd* cc, int with_wabej);
 int aq_rhfccsd_tensor(const aq_rhfccsd* cc, int which, const aq_tensor** out);
 /* Precision the next iteration's contractions will use. */
 int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision);
+/* HIP-graph counters of the contraction stage (aq_rhfccsd_config.graph):
+ * graphs captured, replays, and abandoned captures.  No reference counterpart
+ * (TaskDAG::execute, src/task/task.cxx:495-588, runs every task eagerly). */
+int aq_rhfccsd_graph_stats(const aq_rhfccsd* cc, int* captures, int* replays, int* failed);
 
 /* ---- synthetic closed-shell integrals (workload construction) ---------- */
 
