@@ -52,6 +52,7 @@ struct GemmArgs {
     double* P;   // split-K partials [split][batch][M*N], or null
     int batch;
     int transA, transB;
+    int off32;   // every per-thread tile offset of A and B fits 32 bits (FP64/FP32 interior fast path)
 };
 
 __device__ __forceinline__ int64_t remap_tile(int64_t bid, int64_t nb) {
@@ -91,8 +92,10 @@ __device__ __forceinline__ void tile_coord(int e, int& row, int& kk) {
 // thread ROWS*BK/256 elements).  KC: the operand is stored k-contiguous
 // (element (row,k) at row*ld + k), otherwise row-contiguous (k*ld + row).
 // Each thread keeps one base pointer, a fixed coordinate and a validity mask;
-// out-of-range elements read an in-bounds clamped address and are zeroed, so
-// the loads carry no branches.
+// out-of-range elements read an in-bounds clamped address and are zeroed when
+// the registers are written to LDS, so nothing consumes a loaded value before
+// the MFMAs of the current tile (a select right after the load would make the
+// wave wait for the whole memory latency there).
 template <typename T, int ROWS, int BK, bool KC>
 struct TileLoader {
     static_assert((ROWS * BK) % 256 == 0, "tile must divide over 256 threads");
@@ -103,11 +106,23 @@ struct TileLoader {
     int64_t ld;
     int fixed, var0;
     unsigned rowmask;   // KC: bit r = row of element r in range; !KC: bit 0 = the fixed row in range
+    unsigned okbits;    // bit r: element r of the loaded tile is inside the operand
+    // interior fast path: element r of k-tile kt is at org + kt*kstep (the
+    // buffer resource, uniform) + r*rstep (scalar offset) + this thread's byte
+    // offset boff (vector offset), so a load needs no address arithmetic of its
+    // own and no mask
+    const T* org;
+    int64_t kstep, rstep;
+    uint32_t boff;
     T v[NE];
 
     __device__ __forceinline__ void init(const T* P, int64_t ld_, int64_t row0, int64_t nrows, int64_t kbeg, int tid) {
         ld = ld_;
         safe = P;
+        org = KC ? P + row0 * ld_ + kbeg : P + kbeg * ld_ + row0;
+        kstep = KC ? BK : BK * ld_;
+        rstep = int64_t(VSTEP) * ld_;
+        boff = uint32_t((int64_t(KC ? tid / BK : tid / ROWS) * ld_ + (KC ? tid % BK : tid % ROWS)) * int64_t(sizeof(T)));
         if (KC) {
             fixed = tid % BK;        // k within the tile
             var0 = tid / BK;         // row of element 0
@@ -126,6 +141,7 @@ struct TileLoader {
     // load k-tile kt (k0 = kbeg + kt*BK) into registers
     __device__ __forceinline__ void load(int64_t kt, int64_t kbeg, int64_t kend) {
         const int64_t k0 = kbeg + kt * BK;
+        unsigned bits = 0;
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
             bool ok;
@@ -137,17 +153,36 @@ struct TileLoader {
                 ok = rowmask && (k0 + var0 + r * VSTEP < kend);
                 p = base + (kt * BK + r * VSTEP) * ld;
             }
-            const T x = *(ok ? p : safe);
-            v[r] = ok ? x : T(0);
+            v[r] = *(ok ? p : safe);
+            bits |= unsigned(ok) << r;
         }
+        okbits = bits;
     }
-    // registers -> LDS tile S[k][row]
-    template <int LD>
-    __device__ __forceinline__ void store(T (*S)[LD]) const {
+    // complete k-tile kt of a tile whose rows are all inside the operand; every
+    // offset is below 2^31 bytes (GemmArgs::off32)
+    __device__ __forceinline__ void load_fast(int64_t kt) {
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)(org + kt * kstep), (short)0, 0x7fffffff, 0x00020000);
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
-            if (KC) S[fixed][var0 + r * VSTEP] = v[r];
-            else S[var0 + r * VSTEP][fixed] = v[r];
+            // uniform by construction; readfirstlane keeps it in a scalar register
+            const int so = __builtin_amdgcn_readfirstlane(int(int64_t(r) * rstep * int64_t(sizeof(T))));
+            if constexpr (sizeof(T) == 8)
+                v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, boff, so, 0));
+            else
+                v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, so, 0));
+        }
+        okbits = ~0u;
+    }
+    // registers -> LDS tile S[k][row], out-of-range elements as zeros unless
+    // the tile was loaded by load_fast (`all`, uniform)
+    template <int LD>
+    __device__ __forceinline__ void store(T (*S)[LD], bool all) const {
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            const T x = (all || ((okbits >> r) & 1u)) ? v[r] : T(0);
+            if (KC) S[fixed][var0 + r * VSTEP] = x;
+            else S[var0 + r * VSTEP][fixed] = x;
         }
     }
 };
@@ -181,13 +216,28 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
     TileLoader<T, BN, BK, !TB> lb;
     la.init(static_cast<const T*>(g.A) + int64_t(b) * g.strideA, g.lda, m0, g.M, kbeg, tid);
     lb.init(static_cast<const T*>(g.B) + int64_t(b) * g.strideB, g.ldb, n0, g.N, kbeg, tid);
+    // tiles inside both operands load their complete k-tiles unmasked
+    const bool interior = g.off32 && m0 + BM <= g.M && n0 + BN <= g.N;
+    const int64_t nfull = kend > kbeg ? (kend - kbeg) / BK : 0;
+    bool staged_fast = false;   // the registers hold a load_fast tile
     auto load = [&](int64_t kt) {
-        la.load(kt, kbeg, kend);
-        lb.load(kt, kbeg, kend);
+        staged_fast = interior && kt < nfull;
+        if (staged_fast) {
+            la.load_fast(kt);
+            lb.load_fast(kt);
+        } else {
+            la.load(kt, kbeg, kend);
+            lb.load(kt, kbeg, kend);
+        }
     };
     auto store = [&](int buf) {
-        la.store(As[buf]);
-        lb.store(Bs[buf]);
+        if (staged_fast) {
+            la.store(As[buf], true);
+            lb.store(Bs[buf], true);
+        } else {
+            la.store(As[buf], false);
+            lb.store(Bs[buf], false);
+        }
     };
 
     acc_t acc[FM][FN];
@@ -613,6 +663,8 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     a.batch = int(d.batch);
     a.transA = d.transA; a.transB = d.transB;
     a.P = nullptr;
+    // an element of a tile lies below 144 * ld + 256 elements from the tile origin
+    a.off32 = (144 * std::max(d.lda, d.ldb) + 256) * 8 < (int64_t(1) << 31) ? 1 : 0;
 
     int variant = ctx->gemm_variant[d.fmt];
     if (variant < 0) variant = ctx->autotune ? tuned_variant(ctx, d, BM, BN, a) : default_gemm_variant(d.fmt);
