@@ -107,22 +107,11 @@ struct TileLoader {
     int fixed, var0;
     unsigned rowmask;   // KC: bit r = row of element r in range; !KC: bit 0 = the fixed row in range
     unsigned okbits;    // bit r: element r of the loaded tile is inside the operand
-    // interior fast path: element r of k-tile kt is at org + kt*kstep (the
-    // buffer resource, uniform) + r*rstep (scalar offset) + this thread's byte
-    // offset boff (vector offset), so a load needs no address arithmetic of its
-    // own and no mask
-    const T* org;
-    int64_t kstep, rstep;
-    uint32_t boff;
     T v[NE];
 
     __device__ __forceinline__ void init(const T* P, int64_t ld_, int64_t row0, int64_t nrows, int64_t kbeg, int tid) {
         ld = ld_;
         safe = P;
-        org = KC ? P + row0 * ld_ + kbeg : P + kbeg * ld_ + row0;
-        kstep = KC ? BK : BK * ld_;
-        rstep = int64_t(VSTEP) * ld_;
-        boff = uint32_t((int64_t(KC ? tid / BK : tid / ROWS) * ld_ + (KC ? tid % BK : tid % ROWS)) * int64_t(sizeof(T)));
         if (KC) {
             fixed = tid % BK;        // k within the tile
             var0 = tid / BK;         // row of element 0
@@ -158,9 +147,43 @@ struct TileLoader {
         }
         okbits = bits;
     }
-    // complete k-tile kt of a tile whose rows are all inside the operand; every
-    // offset is below 2^31 bytes (GemmArgs::off32)
-    __device__ __forceinline__ void load_fast(int64_t kt) {
+    // registers -> LDS tile S[k][row], out-of-range elements as zeros
+    template <int LD>
+    __device__ __forceinline__ void store(T (*S)[LD]) const {
+#pragma unroll
+        for (int r = 0; r < NE; ++r) {
+            const T x = ((okbits >> r) & 1u) ? v[r] : T(0);
+            if (KC) S[fixed][var0 + r * VSTEP] = x;
+            else S[var0 + r * VSTEP][fixed] = x;
+        }
+    }
+};
+
+// Loader of complete k-tiles of a tile whose rows all lie inside the operand
+// (same element assignment as TileLoader).  Element r of k-tile kt is at
+// org + kt*kstep (the buffer resource, uniform) + r*rstep (scalar offset) +
+// this thread's byte offset boff (vector offset): a load needs no address
+// arithmetic of its own and no mask.  Every offset is below 2^31 bytes
+// (GemmArgs::off32).  The caller owns the staging registers, so several
+// k-tiles can be in flight.
+template <typename T, int ROWS, int BK, bool KC>
+struct FastLoader {
+    static constexpr int NE = ROWS * BK / 256;
+    static constexpr int VSTEP = 256 / (KC ? BK : ROWS);
+    const T* org;
+    int64_t kstep, rstep;
+    uint32_t boff;
+    int fixed, var0;
+
+    __device__ __forceinline__ void init(const T* P, int64_t ld, int64_t row0, int64_t kbeg, int tid) {
+        fixed = KC ? tid % BK : tid % ROWS;
+        var0 = KC ? tid / BK : tid / ROWS;
+        org = KC ? P + row0 * ld + kbeg : P + kbeg * ld + row0;
+        kstep = KC ? BK : BK * ld;
+        rstep = int64_t(VSTEP) * ld;
+        boff = uint32_t((int64_t(var0) * ld + fixed) * int64_t(sizeof(T)));
+    }
+    __device__ __forceinline__ void load(int64_t kt, T (&v)[NE]) const {
         const __amdgpu_buffer_rsrc_t rs =
             __builtin_amdgcn_make_buffer_rsrc((void*)(org + kt * kstep), (short)0, 0x7fffffff, 0x00020000);
 #pragma unroll
@@ -172,17 +195,13 @@ struct TileLoader {
             else
                 v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, so, 0));
         }
-        okbits = ~0u;
     }
-    // registers -> LDS tile S[k][row], out-of-range elements as zeros unless
-    // the tile was loaded by load_fast (`all`, uniform)
     template <int LD>
-    __device__ __forceinline__ void store(T (*S)[LD], bool all) const {
+    __device__ __forceinline__ void store(T (*S)[LD], const T (&v)[NE]) const {
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
-            const T x = (all || ((okbits >> r) & 1u)) ? v[r] : T(0);
-            if (KC) S[fixed][var0 + r * VSTEP] = x;
-            else S[var0 + r * VSTEP][fixed] = x;
+            if (KC) S[fixed][var0 + r * VSTEP] = v[r];
+            else S[var0 + r * VSTEP][fixed] = v[r];
         }
     }
 };
@@ -211,51 +230,14 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
     const int64_t kbeg = int64_t(split) * g.kchunk;
     const int64_t kend = min(g.K, kbeg + g.kchunk);
 
-    // opA(m,k) = A[k + m*lda] when TA (k contiguous); opB(k,n) = B[k + n*ldb] when !TB (k contiguous)
-    TileLoader<T, BM, BK, TA> la;
-    TileLoader<T, BN, BK, !TB> lb;
-    la.init(static_cast<const T*>(g.A) + int64_t(b) * g.strideA, g.lda, m0, g.M, kbeg, tid);
-    lb.init(static_cast<const T*>(g.B) + int64_t(b) * g.strideB, g.ldb, n0, g.N, kbeg, tid);
-    // tiles inside both operands load their complete k-tiles unmasked
-    const bool interior = g.off32 && m0 + BM <= g.M && n0 + BN <= g.N;
-    const int64_t nfull = kend > kbeg ? (kend - kbeg) / BK : 0;
-    bool staged_fast = false;   // the registers hold a load_fast tile
-    auto load = [&](int64_t kt) {
-        staged_fast = interior && kt < nfull;
-        if (staged_fast) {
-            la.load_fast(kt);
-            lb.load_fast(kt);
-        } else {
-            la.load(kt, kbeg, kend);
-            lb.load(kt, kbeg, kend);
-        }
-    };
-    auto store = [&](int buf) {
-        if (staged_fast) {
-            la.store(As[buf], true);
-            lb.store(Bs[buf], true);
-        } else {
-            la.store(As[buf], false);
-            lb.store(Bs[buf], false);
-        }
-    };
-
     acc_t acc[FM][FN];
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    if (nk > 0) load(0);
-    for (int64_t kt = 0; kt < nk; ++kt) {
-        const int buf = int(kt & 1);
-        store(buf);
-        __syncthreads();
-        // next tile's global loads go out before this tile's MFMAs, so their
-        // latency hides behind the matrix work
-        if (kt + 1 < nk) load(kt + 1);
-        __builtin_amdgcn_sched_barrier(0);
+    // one k-tile of MFMAs from LDS buffer buf
+    auto compute = [&](int buf) {
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + lane / 16;
@@ -268,6 +250,69 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
             for (int i = 0; i < FM; ++i)
 #pragma unroll
                 for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::run(x[j], y[i], acc[i][j]);
+        }
+    };
+
+    // opA(m,k) = A[k + m*lda] when TA (k contiguous); opB(k,n) = B[k + n*ldb] when !TB (k contiguous)
+    const T* Ab = static_cast<const T*>(g.A) + int64_t(b) * g.strideA;
+    const T* Bb = static_cast<const T*>(g.B) + int64_t(b) * g.strideB;
+    const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
+    const int64_t nfull = kend > kbeg ? (kend - kbeg) / BK : 0;
+    if (g.off32 && m0 + BM <= g.M && n0 + BN <= g.N) {
+        // Interior tile: complete k-tiles through FastLoader, two k-tiles in
+        // flight (the tile stored now was loaded two steps ago), so a load
+        // has two tiles of MFMA work to arrive.  Register set s and LDS
+        // buffer s serve the k-tiles of parity s.
+        FastLoader<T, BM, BK, TA> fa;
+        FastLoader<T, BN, BK, !TB> fb;
+        fa.init(Ab, g.lda, m0, kbeg, tid);
+        fb.init(Bb, g.ldb, n0, kbeg, tid);
+        T va[2][FastLoader<T, BM, BK, TA>::NE], vb[2][FastLoader<T, BN, BK, !TB>::NE];
+        if (nfull > 0) { fa.load(0, va[0]); fb.load(0, vb[0]); }
+        if (nfull > 1) { fa.load(1, va[1]); fb.load(1, vb[1]); }
+        auto step = [&](auto S, int64_t kt) {
+            constexpr int s = decltype(S)::value;
+            fa.store(As[s], va[s]);
+            fb.store(Bs[s], vb[s]);
+            __syncthreads();
+            if (kt + 2 < nfull) { fa.load(kt + 2, va[s]); fb.load(kt + 2, vb[s]); }
+            __builtin_amdgcn_sched_barrier(0);
+            compute(s);
+        };
+        for (int64_t kt = 0; kt < nfull; kt += 2) {
+            step(std::integral_constant<int, 0>{}, kt);
+            if (kt + 1 < nfull) step(std::integral_constant<int, 1>{}, kt + 1);
+        }
+        if (nk > nfull) {   // the partial last k-tile
+            TileLoader<T, BM, BK, TA> la;
+            TileLoader<T, BN, BK, !TB> lb;
+            la.init(Ab, g.lda, m0, g.M, kbeg, tid);
+            lb.init(Bb, g.ldb, n0, g.N, kbeg, tid);
+            la.load(nfull, kbeg, kend);
+            lb.load(nfull, kbeg, kend);
+            const int buf = int(nfull & 1);
+            la.store(As[buf]);
+            lb.store(Bs[buf]);
+            __syncthreads();
+            compute(buf);
+        }
+    } else {
+        // edge tile: predicated loads, one k-tile in flight
+        TileLoader<T, BM, BK, TA> la;
+        TileLoader<T, BN, BK, !TB> lb;
+        la.init(Ab, g.lda, m0, g.M, kbeg, tid);
+        lb.init(Bb, g.ldb, n0, g.N, kbeg, tid);
+        if (nk > 0) { la.load(0, kbeg, kend); lb.load(0, kbeg, kend); }
+        for (int64_t kt = 0; kt < nk; ++kt) {
+            const int buf = int(kt & 1);
+            la.store(As[buf]);
+            lb.store(Bs[buf]);
+            __syncthreads();
+            // next tile's global loads go out before this tile's MFMAs, so their
+            // latency hides behind the matrix work
+            if (kt + 1 < nk) { la.load(kt + 1, kbeg, kend); lb.load(kt + 1, kbeg, kend); }
+            __builtin_amdgcn_sched_barrier(0);
+            compute(buf);
         }
     }
 
