@@ -345,7 +345,14 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
 template <int BM, int BN, int WM, int WN, int PF>
 __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g) {
     constexpr int BK = 32;
-    constexpr int KP = BK + 8;                      // padded row (bf16 elements)
+    // LDS rows of BK bf16 (64 bytes, four 16-byte chunks), chunk c of row r
+    // stored at chunk c ^ ((r >> 1) & 3).  Measured on gfx950 (bank-conflict
+    // counter): this layout serves both the MFMA operand reads (lane -> row
+    // lane%16, chunk lane/16) and the tile stores (lane -> row lane/4, chunk
+    // lane%4) without bank conflicts; an 80-byte padded row costs 4 conflict
+    // cycles per ds_read_b128 and 8 per ds_write_b128.
+    constexpr int KP = BK;
+    auto swz = [](int row, int k) { return (((k >> 3) ^ (row >> 1)) & 3) << 3; };
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     constexpr int VA = BM * BK / 8, VB = BN * BK / 8;   // 8-element vectors per plane
@@ -411,16 +418,16 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
             const int e = tid + 256 * r;
             if (e >= VA) continue;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
-            *reinterpret_cast<i16x8*>(&As[buf][0][mm][kk]) = xa[0][r];
-            *reinterpret_cast<i16x8*>(&As[buf][1][mm][kk]) = xa[1][r];
+            *reinterpret_cast<i16x8*>(&As[buf][0][mm][swz(mm, kk)]) = xa[0][r];
+            *reinterpret_cast<i16x8*>(&As[buf][1][mm][swz(mm, kk)]) = xa[1][r];
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
             const int e = tid + 256 * r;
             if (e >= VB) continue;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
-            *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][kk]) = xb[0][r];
-            *reinterpret_cast<i16x8*>(&Bs[buf][1][nn][kk]) = xb[1][r];
+            *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][swz(nn, kk)]) = xb[0][r];
+            *reinterpret_cast<i16x8*>(&Bs[buf][1][nn][swz(nn, kk)]) = xb[1][r];
         }
     };
 
@@ -438,14 +445,14 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
             const int nn = wn * WTN + j * 16 + lane % 16;
-            xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][kq]));
-            xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][kq]));
+            xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][swz(nn, kq)]));
+            xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][swz(nn, kq)]));
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
             const int mm = wm * WTM + i * 16 + lane % 16;
-            yh[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][kq]));
-            yl[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][kq]));
+            yh[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][swz(mm, kq)]));
+            yl[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][swz(mm, kq)]));
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i)
