@@ -33,7 +33,8 @@ class RHFCCSDConfig(C.Structure):
     _fields_ = [("convergence", C.c_double), ("max_iterations", C.c_int), ("diis_order", C.c_int),
                 ("diis_start", C.c_int), ("diis_damping", C.c_double), ("precision", C.c_int),
                 ("refine_below", C.c_double), ("min_reduced_flops", C.c_double), ("fused_update", C.c_int),
-                ("diis_reset_on_refine", C.c_int), ("fp32_stage_below", C.c_double), ("graph", C.c_int)]
+                ("diis_reset_on_refine", C.c_int), ("fp32_stage_below", C.c_double), ("graph", C.c_int),
+                ("packed_ladder", C.c_int)]
 
 
 class TuneEntry(C.Structure):
@@ -96,6 +97,7 @@ _SIGS = {
     "aq_rhfccsd_hbar": [_P, C.c_int],
     "aq_rhfccsd_current_precision": [_P, C.POINTER(C.c_int)],
     "aq_rhfccsd_graph_stats": [_P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "aq_rhfccsd_packed_ladder": [_P, C.POINTER(C.c_int)],
     "aq_synth_host": [C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_double, _DP, _DP],
     "aq_synth_default_sigma": [C.c_int, C.c_int],
     "aq_synth_build": [_P, C.c_int, C.c_int, C.c_int, _DP, _DP, C.POINTER(_P)],
@@ -423,6 +425,13 @@ class RHFCCSD:
         r = C.c_int()
         _ck(lib().aq_rhfccsd_current_precision(self.h, C.byref(r)))
         return r.value
+
+    @property
+    def packed_ladder(self):
+        """True when the ladder runs on packed (anti)symmetric pairs."""
+        r = C.c_int()
+        _ck(lib().aq_rhfccsd_packed_ladder(self.h, C.byref(r)))
+        return bool(r.value)
 
     def graph_stats(self):
         """(captures, replays, abandoned captures) of the contraction-stage HIP graph."""

@@ -518,6 +518,150 @@ void launch_ccsd_update(aq_ctx* ctx, const UpdateArgs& u, int slot_e1, int slot_
     finish(ctx, slot_c2, g2, RED_MAX);
 }
 
+// ------------------------------------------------------- packed ladder
+//
+// The particle-particle ladder S(ab,ij) = sum_ef V(ab,ef) Tau(ef,ij) with
+// V(ab,ef) = V(ba,fe) and Tau(ef,ij) = Tau(fe,ji) splits into a part
+// symmetric and a part antisymmetric under e<->f (Scuseria, Janssen, Schaefer,
+// J. Chem. Phys. 89, 7382 (1988)):
+//   S+(ab,ij) = sum_{e<=f} V+(ab,ef) T+(ef,ij),  a<=b, i<=j
+//   S-(ab,ij) = sum_{e<f}  V-(ab,ef) T-(ef,ij),  a<b,  i<j
+//   V+(ab,ef) = V(ab,ef) + V(ab,fe) (V(ab,ee) on the diagonal), V- = V(ab,ef) - V(ab,fe)
+//   T+-(ef,ij) = (Tau(ef,ij) +- Tau(fe,ij)) / 2
+//   S(ab,ij) = S+(ab,ij) + S-(ab,ij), S+ symmetric and S- antisymmetric
+//   under a<->b and under i<->j.
+// Packed pair indices: tri_le(a,b) = a + b(b+1)/2 for a <= b,
+// tri_lt(a,b) = a + b(b-1)/2 for a < b (column b holds the rows a).
+
+__device__ __forceinline__ int64_t tri_le(int64_t a, int64_t b) { return a + b * (b + 1) / 2; }
+__device__ __forceinline__ int64_t tri_lt(int64_t a, int64_t b) { return a + b * (b - 1) / 2; }
+
+// inverse of tri_le: r -> (a, b), a <= b
+__device__ __forceinline__ void untri_le(int64_t r, int64_t& a, int64_t& b) {
+    int64_t bb = int64_t((sqrt(8.0 * double(r) + 1.0) - 1.0) * 0.5);
+    while (bb * (bb + 1) / 2 > r) --bb;
+    while ((bb + 1) * (bb + 2) / 2 <= r) ++bb;
+    b = bb;
+    a = r - bb * (bb + 1) / 2;
+}
+
+// V+ [P x P] and V- [Q x Q] (column-major) from V[v,v,v,v] (a fastest);
+// grid.y walks the (e <= f) columns, threads the (a <= b) rows.
+__global__ __launch_bounds__(TPB) void k_ladder_pack_v(const double* __restrict__ V, int64_t v, double* __restrict__ Vp,
+                                                       double* __restrict__ Vm) {
+    const int64_t P = v * (v + 1) / 2, Q = v * (v - 1) / 2;
+    for (int64_t c = blockIdx.y; c < P; c += gridDim.y) {
+        int64_t e, f;
+        untri_le(c, e, f);
+        const double* Vef = V + (e + f * v) * v * v;
+        const double* Vfe = V + (f + e * v) * v * v;
+        for (int64_t r = int64_t(blockIdx.x) * TPB + threadIdx.x; r < P; r += int64_t(gridDim.x) * TPB) {
+            int64_t a, b;
+            untri_le(r, a, b);
+            const double x = Vef[a + b * v], y = Vfe[a + b * v];
+            Vp[r + c * P] = e == f ? x : x + y;
+            if (a < b && e < f) Vm[tri_lt(a, b) + tri_lt(e, f) * Q] = x - y;
+        }
+    }
+}
+
+// T+ [P x O+] and T- [Q x O-] from Tau[v,v,o,o]; grid.y walks the (i <= j) columns.
+__global__ __launch_bounds__(TPB) void k_ladder_pack_t(const double* __restrict__ T, int64_t v, int64_t o,
+                                                       double* __restrict__ Tp, double* __restrict__ Tm) {
+    const int64_t P = v * (v + 1) / 2, Q = v * (v - 1) / 2, Op = o * (o + 1) / 2;
+    for (int64_t c = blockIdx.y; c < Op; c += gridDim.y) {
+        int64_t i, j;
+        untri_le(c, i, j);
+        const double* Tij = T + (i + j * o) * v * v;
+        for (int64_t r = int64_t(blockIdx.x) * TPB + threadIdx.x; r < P; r += int64_t(gridDim.x) * TPB) {
+            int64_t e, f;
+            untri_le(r, e, f);
+            const double x = Tij[e + f * v], y = Tij[f + e * v];
+            Tp[r + c * P] = 0.5 * (x + y);
+            if (e < f && i < j) Tm[tri_lt(e, f) + tri_lt(i, j) * Q] = 0.5 * (x - y);
+        }
+    }
+}
+
+// Z[v,v,o,o] += alpha * (S+ + S-) expanded to all (a,b,i,j)
+__global__ __launch_bounds__(TPB) void k_ladder_unpack(const double* __restrict__ Sp, const double* __restrict__ Sm,
+                                                       int64_t v, int64_t o, double alpha, double* __restrict__ Z) {
+    const int64_t P = v * (v + 1) / 2, Q = v * (v - 1) / 2;
+    const int64_t vv = v * v;
+    for (int64_t c = blockIdx.y; c < o * o; c += gridDim.y) {
+        const int64_t i = c % o, j = c / o;
+        const int64_t ilo = i < j ? i : j, ihi = i < j ? j : i;
+        const double sij = i < j ? 1.0 : -1.0;
+        const double* Spc = Sp + tri_le(ilo, ihi) * P;
+        const double* Smc = i != j ? Sm + tri_lt(ilo, ihi) * Q : nullptr;
+        double* Zc = Z + c * vv;
+        for (int64_t r = int64_t(blockIdx.x) * TPB + threadIdx.x; r < vv; r += int64_t(gridDim.x) * TPB) {
+            const int64_t a = r % v, b = r / v;
+            const int64_t alo = a < b ? a : b, ahi = a < b ? b : a;
+            double s = Spc[tri_le(alo, ahi)];
+            if (Smc && a != b) s += (a < b ? sij : -sij) * Smc[tri_lt(alo, ahi)];
+            Zc[r] += alpha * s;
+        }
+    }
+}
+
+// max |V(a,b,e,f) - V(b,a,f,e)| into result slot `slot` (caller syncs)
+__global__ __launch_bounds__(TPB) void k_pair_asym(const double* __restrict__ X, int64_t n1, int64_t n2,
+                                                   double* partial) {
+    // X[n1,n1,n2,n2]: deviation from X(p,q,r,s) = X(q,p,s,r)
+    const int64_t n = n1 * n1 * n2 * n2;
+    double acc = 0.0;
+    for (int64_t k = int64_t(blockIdx.x) * TPB + threadIdx.x; k < n; k += int64_t(gridDim.x) * TPB) {
+        int64_t t = k;
+        const int64_t p = t % n1; t /= n1;
+        const int64_t q = t % n1; t /= n1;
+        const int64_t r = t % n2, s = t / n2;
+        acc = fmax(acc, fabs(X[k] - X[q + n1 * (p + n1 * (s + n2 * r))]));
+    }
+    const double m = block_reduce(acc, RED_MAX);
+    if (threadIdx.x == 0) partial[blockIdx.x] = m;
+}
+
+void launch_ladder_pack_v(aq_ctx* ctx, const double* V, int v, double* Vp, double* Vm) {
+    const int64_t P = int64_t(v) * (v + 1) / 2;
+    TimedLaunch tl(ctx, "ladder_pack_v", 0.0, 8.0 * (2.0 * P * P + 2.0 * P * P));
+    const unsigned gx = unsigned(std::min<int64_t>((P + TPB - 1) / TPB, 64));
+    const unsigned gy = unsigned(std::min<int64_t>(P, 65535));
+    hipLaunchKernelGGL(k_ladder_pack_v, dim3(gx, gy), dim3(TPB), 0, ctx->stream, V, int64_t(v), Vp, Vm);
+    AQ_HIP(hipGetLastError());
+}
+
+void launch_ladder_pack_t(aq_ctx* ctx, const double* T, int v, int o, double* Tp, double* Tm) {
+    const int64_t P = int64_t(v) * (v + 1) / 2, Op = int64_t(o) * (o + 1) / 2;
+    TimedLaunch tl(ctx, "ladder_pack_t", 0.0, 8.0 * (2.0 * P * Op + 2.0 * P * Op));
+    const unsigned gx = unsigned(std::min<int64_t>((P + TPB - 1) / TPB, 64));
+    const unsigned gy = unsigned(std::min<int64_t>(Op, 65535));
+    hipLaunchKernelGGL(k_ladder_pack_t, dim3(gx, gy), dim3(TPB), 0, ctx->stream, T, int64_t(v), int64_t(o), Tp, Tm);
+    AQ_HIP(hipGetLastError());
+}
+
+void launch_ladder_unpack(aq_ctx* ctx, const double* Sp, const double* Sm, int v, int o, double alpha, double* Z) {
+    const int64_t vv = int64_t(v) * v, oo = int64_t(o) * o;
+    TimedLaunch tl(ctx, "ladder_unpack", 0.0, 8.0 * 3.0 * vv * oo);
+    const unsigned gx = unsigned(std::min<int64_t>((vv + TPB - 1) / TPB, 64));
+    const unsigned gy = unsigned(std::min<int64_t>(oo, 65535));
+    hipLaunchKernelGGL(k_ladder_unpack, dim3(gx, gy), dim3(TPB), 0, ctx->stream, Sp, Sm, int64_t(v), int64_t(o),
+                       alpha, Z);
+    AQ_HIP(hipGetLastError());
+}
+
+double pair_asymmetry(aq_ctx* ctx, const double* X, int n1, int n2, int slot) {
+    AQ_CHECK(slot >= 0 && slot < 8, AQ_ERR_INVALID, "reduction slot out of range");
+    const int64_t n = int64_t(n1) * n1 * n2 * n2;
+    if (n == 0) return 0.0;
+    const unsigned g = red_blocks(n);
+    hipLaunchKernelGGL(k_pair_asym, dim3(g), dim3(TPB), 0, ctx->stream, X, int64_t(n1), int64_t(n2),
+                       partial_slot(ctx, slot));
+    AQ_HIP(hipGetLastError());
+    finish(ctx, slot, g, RED_MAX);
+    return sync_result(ctx, slot);
+}
+
 // ------------------------------------------------------------------ DIIS
 
 constexpr int MAXK = 16;

@@ -229,6 +229,15 @@ struct UpdateArgs {
 void launch_ccsd_update(aq_ctx* ctx, const UpdateArgs& a, int slot_e1, int slot_e2, int slot_c1,
                         int slot_c2);
 
+// packed particle-particle ladder (elementwise.hip): V+- [P x P], [Q x Q] and
+// T+- [P x O+], [Q x O-] with P = v(v+1)/2, Q = v(v-1)/2, O+ = o(o+1)/2,
+// O- = o(o-1)/2; Z[v,v,o,o] += alpha (S+ + S-)
+void launch_ladder_pack_v(aq_ctx* ctx, const double* V, int v, double* Vp, double* Vm);
+void launch_ladder_pack_t(aq_ctx* ctx, const double* T, int v, int o, double* Tp, double* Tm);
+void launch_ladder_unpack(aq_ctx* ctx, const double* Sp, const double* Sm, int v, int o, double alpha, double* Z);
+// max |X(p,q,r,s) - X(q,p,s,r)| of X[n1,n1,n2,n2] (synchronising)
+double pair_asymmetry(aq_ctx* ctx, const double* X, int n1, int n2, int slot);
+
 // DIIS helpers (elementwise.hip)
 void launch_multi_dot(aq_ctx* ctx, const double* x, const double* const* ys, int k, int64_t n,
                       int first_slot);

@@ -40,6 +40,9 @@ struct aq_rhfccsd {
     Tensor T1, T2, Z1, Z2, Tau, T2SA, TauSA, VABIJSA, VAIJBSA;
     Tensor FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE;
     Tensor WABEJ;                       // formed by hbar(true) only
+    // packed ladder: V+- (constant), T+- and S+- per iteration
+    bool packed = false;
+    Tensor Vp, Vm, Tp, Tm, Sp, Sm;
     std::vector<double> dA, dI;
     Tensor d_dA, d_dI;
     std::unique_ptr<DIIS> diis;
@@ -114,6 +117,28 @@ struct aq_rhfccsd {
         VAIJBSA["aijb"] -= VAIBJ["aibj"];              // :92
 
         mp2 = 2 * scalar(fAI["ai"] * T1["ai"]) + scalar(VABIJSA["abij"] * Tau["abij"]);   // :94
+
+        // The ladder can run on the (anti)symmetric packed forms when
+        // <ab|ef> = <ba|fe> and <ab|ij> = <ba|ji> (so Tau(ef,ij) = Tau(fe,ji)).
+        if (cfg.packed_ladder && o >= 2 && v >= 2) {
+            const double dV = pair_asymmetry(ctx, dptr(VABCD), v, v, 0);
+            const double dI = pair_asymmetry(ctx, dptr(VABIJ), v, o, 0);
+            const double nV = VABCD.norm(0), nI = VABIJ.norm(0);
+            packed = dV <= 1e-13 * (1.0 + nV) && dI <= 1e-13 * (1.0 + nI);
+        }
+        if (packed) {
+            const int64_t P = int64_t(v) * (v + 1) / 2, Q = int64_t(v) * (v - 1) / 2;
+            const int64_t Op = int64_t(o) * (o + 1) / 2, Om = int64_t(o) * (o - 1) / 2;
+            Vp = mk({P, P});
+            Vm = mk({Q, Q});
+            Tp = mk({P, Op});
+            Tm = mk({Q, Om});
+            Sp = mk({P, Op});
+            Sm = mk({Q, Om});
+            launch_ladder_pack_v(ctx, dptr(VABCD), v, dptr(Vp), dptr(Vm));
+            bump_t(Vp);
+            bump_t(Vm);
+        }
         diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));   // :98
     }
 
@@ -180,7 +205,17 @@ struct aq_rhfccsd {
         Z2["abij"] -= WAMIJ["amij"] * T1["bm"];                    // :300
         Z2["abij"] += FAE["ae"] * T2["ebij"];                      // :301
         Z2["abij"] -= FMI["mi"] * T2["abmj"];                      // :302
-        Z2["abij"] += 0.5 * VABCD["abef"] * Tau["efij"];           // :303  particle-particle ladder
+        if (packed) {                                              // :303  particle-particle ladder,
+            launch_ladder_pack_t(ctx, dptr(Tau), v, o, dptr(Tp), dptr(Tm));   // as S+ + S- on packed pairs
+            bump_t(Tp);
+            bump_t(Tm);
+            Sp["pi"] = Vp["pq"] * Tp["qi"];
+            Sm["pi"] = Vm["pq"] * Tm["qi"];
+            launch_ladder_unpack(ctx, dptr(Sp), dptr(Sm), v, o, 0.5, dptr(Z2));
+            bump_t(Z2);
+        } else {
+            Z2["abij"] += 0.5 * VABCD["abef"] * Tau["efij"];       // :303  particle-particle ladder
+        }
         Z2["abij"] += 0.5 * WMNIJ["mnij"] * Tau["abmn"];           // :304
         Z2["abij"] += 0.5 * WAMIE["amie"] * T2SA["ebmj"];          // :305  ring
         Z2["abij"] -= 0.5 * WAMEI["amei"] * T2["ebjm"];            // :306  ring
@@ -398,6 +433,7 @@ int aq_rhfccsd_default_config(aq_rhfccsd_config* c) {
         c->diis_reset_on_refine = 0;
         c->fp32_stage_below = 0.0;
         c->graph = 0;
+        c->packed_ladder = 1;
     });
 }
 
@@ -527,6 +563,13 @@ int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision) {
     return guarded([&] {
         AQ_CHECK(cc && precision, AQ_ERR_INVALID, "null argument");
         *precision = cc->next_precision();
+    });
+}
+
+int aq_rhfccsd_packed_ladder(const aq_rhfccsd* cc, int* packed) {
+    return guarded([&] {
+        AQ_CHECK(cc && packed, AQ_ERR_INVALID, "null argument");
+        *packed = cc->packed ? 1 : 0;
     });
 }
 

@@ -176,7 +176,8 @@ size_t aq_job_bytes_estimate(int o, int v, int nQ, int diis_order, int precision
     const double O = o, V = v, Q = nQ, k = std::max(diis_order, 1);
     const double V4 = V * V * V * V, V3 = V * V * V * O, N2 = O * O * V * V, O3V = O * O * O * V, O4 = O * O * O * O;
     const double ints = V4 + V3 + 3 * N2 + O3V + O4 + V * V + O * O + V * O;
-    const double solver = 9 * N2 + 2 * O3V + O4 + 4 * V * V;
+    // + the packed ladder: V+- (~V^4 / 2) and T+-, S+- (~N2 together)
+    const double solver = 9 * N2 + 2 * O3V + O4 + 4 * V * V + 0.5 * V4 + N2;
     const double diis = 2 * k * (N2 + V * O);
     const double cache64 = 3 * V3 + 12 * N2;
     const double cacheR = precision != AQ_PREC_FP64 ? 0.5 * (V4 + 3 * V3 + 12 * N2) : 0.0;

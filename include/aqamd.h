@@ -227,6 +227,14 @@ typedef struct aq_rhfccsd_config {
      * path, so the results are bitwise identical.  Ignored while the context
      * records kernel timings.  0: every iteration eager. */
     int graph;
+    /* 1: evaluate the particle-particle ladder (rhfccsd.cxx:303) on the
+     * symmetric and antisymmetric combinations over packed pairs a<=b, e<=f,
+     * i<=j (two GEMMs with a quarter of the FLOPs of the full contraction;
+     * Scuseria, Janssen, Schaefer, J. Chem. Phys. 89, 7382 (1988)).  Needs
+     * <ab|ef> = <ba|fe> and <ab|ij> = <ba|ji>, checked at setup to 1e-13
+     * relative; otherwise, or when 0, the full contraction as the reference
+     * issues it.  Costs (v(v+1)/2)^2 + (v(v-1)/2)^2 doubles of device memory. */
+    int packed_ladder;
 } aq_rhfccsd_config;
 
 int aq_rhfccsd_default_config(aq_rhfccsd_config* cfg);
@@ -269,6 +277,9 @@ int aq_rhfccsd_current_precision(const aq_rhfccsd* cc, int* precision);
  * graphs captured, replays, and abandoned captures.  No reference counterpart
  * (TaskDAG::execute, src/task/task.cxx:495-588, runs every task eagerly). */
 int aq_rhfccsd_graph_stats(const aq_rhfccsd* cc, int* captures, int* replays, int* failed);
+/* 1 when the solver evaluates the ladder on packed pairs (see
+ * aq_rhfccsd_config.packed_ladder), 0 when it runs the full contraction. */
+int aq_rhfccsd_packed_ladder(const aq_rhfccsd* cc, int* packed);
 
 /* ---- synthetic closed-shell integrals (workload construction) ---------- */
 
