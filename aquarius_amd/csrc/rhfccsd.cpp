@@ -430,7 +430,7 @@ int aq_rhfccsd_default_config(aq_rhfccsd_config* c) {
         c->refine_below = 1e-5;
         c->min_reduced_flops = 1e8;
         c->fused_update = 1;
-        c->diis_reset_on_refine = 0;
+        c->diis_reset_on_refine = 1;
         c->fp32_stage_below = 0.0;
         c->graph = 0;
         c->packed_ladder = 1;

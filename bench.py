@@ -45,6 +45,16 @@ def flops_per_iter(o, v):
             + 20 * o**3 * v**2 + 6 * o**2 * v**2 + 2 * o**4 * v + 4 * o**3 * v + 4 * (o**2 * v + o * v**2))
 
 
+def flops_per_iter_executed(o, v, packed):
+    """FLOPs the engine executes per iteration: the packed ladder replaces the
+    2 o^2 v^4 term by two GEMMs over packed pairs (P = v(v+1)/2, Q = v(v-1)/2)."""
+    F = flops_per_iter(o, v)
+    if not packed:
+        return F
+    P, Q, Op, Om = v * (v + 1) // 2, v * (v - 1) // 2, o * (o + 1) // 2, o * (o - 1) // 2
+    return F - 2 * o**2 * v**4 + 2 * P * P * Op + 2 * Q * Q * Om
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -110,7 +120,8 @@ def serving(a):
     import numpy as np
     import aquarius_amd as aq
     jobs = job_stream(a.seed, a.jobs, a.o_min, a.o_max, a.rate)
-    cfg = aq.default_config(convergence=a.convergence, diis_order=a.diis, max_iterations=150, precision=a.mixed,
+    cfg = aq.default_config(packed_ladder=a.packed_ladder, graph=a.graph, convergence=a.convergence,
+                            diis_order=a.diis, max_iterations=150, precision=a.mixed,
                             refine_below=a.refine_below, min_reduced_flops=1e8)
     res = []
     with aq.Server(slots=a.slots, hbm_budget=int(a.budget_gb * 2**30), max_bypass_s=a.max_bypass) as srv:
@@ -144,7 +155,7 @@ def serving(a):
     # one-at-a-time FP64 solves of a sample of the same stream (no arrivals: back to back)
     if a.fp64_sample:
         sample = jobs[::max(1, len(jobs) // a.fp64_sample)][:a.fp64_sample]
-        c64 = aq.default_config(convergence=a.convergence, diis_order=a.diis, max_iterations=150)
+        c64 = aq.default_config(packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=150)
         times = []
         with aq.Server(slots=1, hbm_budget=int(a.budget_gb * 2**30)) as srv:
             for seed, o, v, _ in sample:
@@ -167,7 +178,7 @@ def warm_tuner(a, ctx, ints):
         return
     t0 = time.perf_counter()
     for prec in ("fp64", a.mixed):
-        cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=1,
+        cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=1,
                         precision=prec, refine_below=a.refine_below, min_reduced_flops=1e8)
         cc.iterate()
         cc.close()
@@ -208,7 +219,7 @@ def only(a):
     ints = aq.Integrals.synthetic(ctx, a.seed, o, v, nq)
     ctx.sync()
     prec = "fp64" if a.only == "fp64" else a.mixed
-    cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=1000, precision=prec,
+    cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=1000, precision=prec,
                     refine_below=a.refine_below, min_reduced_flops=1e8)
     for _ in range(a.warmup):
         cc.iterate()
@@ -277,7 +288,7 @@ def molecular(a):
     warm_tuner(a, ctx, ints)
     runs = {}
     for name, prec in (("fp64", "fp64"), ("mixed", a.mixed)):
-        cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=150,
+        cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=150,
                         precision=prec, refine_below=a.refine_below, min_reduced_flops=1e8)
         mp2 = cc.mp2
         t0 = time.perf_counter()
@@ -306,7 +317,9 @@ def molecular(a):
     out = {"metric": METRIC, "value": runs["mixed"]["iter_per_s"], "unit": "iter/s", "n_gpus": 1,
            "higher_is_better": True, "dtype": f"{a.mixed}+fp64", "data": "molecular (integral front end)",
            "config": {"workload": label, "basis": a.basis, "nao": mol.nao, "o": o, "v": v,
-                      "convergence": a.convergence, "diis_order": a.diis, "flops_per_iter": F},
+                      "convergence": a.convergence, "diis_order": a.diis, "flops_per_iter": F,
+                      "packed_ladder": bool(a.packed_ladder),
+                      "flops_per_iter_executed": flops_per_iter_executed(o, v, a.packed_ladder)},
            "scf_energy": scf.energy, "scf_iterations": scf.iterations, "front_end_s": t,
            "fp64": runs["fp64"], "mixed": runs["mixed"],
            "dE_mixed_vs_fp64": runs["mixed"]["energy"] - runs["fp64"]["energy"],
@@ -354,6 +367,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling run: plain device copies of a 1 GiB FP64 tensor (HBM counter calibration)")
     ap.add_argument("--autotune", type=int, default=1, help="GEMM autotuner on (1) or off (0)")
+    ap.add_argument("--packed-ladder", type=int, default=1,
+                    help="particle-particle ladder on packed (anti)symmetric pairs (1) or as the full contraction (0)")
+    ap.add_argument("--graph", type=int, default=1, help="serving: HIP-graph replay of each job's contraction stage")
     ap.add_argument("--compare-precisions", type=int, default=1,
                     help="also solve with the other reduced arithmetic (FP32 MFMA beside BF16x3) and report its "
                          "energy and T2 error against FP64")
@@ -407,7 +423,7 @@ def main():
     # ---- FP64 reference solve (to convergence)
     e64 = None
     if not a.no_fp64:
-        cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=150)
+        cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=150)
         ctx.enable_timing(True)
         ctx.timing_reset()
         t0 = time.perf_counter()
@@ -426,7 +442,7 @@ def main():
         cc.close()
 
     # ---- mixed solve: W warmup, exactly K timed iterations, then on to convergence
-    cc = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
+    cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
                     precision=a.mixed, refine_below=a.refine_below, min_reduced_flops=1e8)
     ctx.enable_timing(True)
     ctx.timing_reset()
@@ -469,7 +485,7 @@ def main():
     other = None
     if a.compare_precisions and e64 is not None:
         prec2 = "fp32" if a.mixed == "bf16x3" else "bf16x3"
-        c2 = aq.RHFCCSD(ctx, ints, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
+        c2 = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
                         precision=prec2, refine_below=a.refine_below, min_reduced_flops=1e8)
         t0 = time.perf_counter()
         r2 = c2.run(callback=lambda h: log(f"{prec2} it {h['iter']:2d} E={h['energy']:.12f} conv={h['conv']:.2e} "
@@ -504,9 +520,12 @@ def main():
         "ms_per_step": 1e3 * t_timed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": f"{a.mixed}+fp64", "data": "synthetic",
         "config": {"workload": f"synthetic closed-shell RHF-CCSD o={o} v={v} nQ={nq} seed={a.seed}, full vvvv "
-                               f"resident; {a.mixed} MFMA contractions until conv<{a.refine_below:g}, then FP64",
+                               f"resident; {a.mixed} MFMA contractions until conv<{a.refine_below:g}, then FP64"
+                               + ("; ladder on packed (anti)symmetric pairs" if a.packed_ladder else ""),
                    "o": o, "v": v, "nQ": nq, "seed": a.seed, "sigma": aq.synth_default_sigma(v, nq),
                    "convergence": a.convergence, "diis_order": a.diis, "flops_per_iter": F,
+                   "packed_ladder": bool(a.packed_ladder),
+                   "flops_per_iter_executed": flops_per_iter_executed(o, v, a.packed_ladder),
                    "gemm_autotune": bool(a.autotune)},
         "roofline": dominant,
         "mixed": {"iterations_to_converge": n_conv, "energy": e_mixed, "time_to_E_s": t_mixed_to_E,

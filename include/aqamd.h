@@ -210,8 +210,10 @@ typedef struct aq_rhfccsd_config {
     /* 1: fuse weight + update + Tau + energy + conv into one kernel;
      * 0: issue them as separate operator calls, as the reference does */
     int fused_update;
-    /* 1: drop the DIIS history when the contractions switch to FP64, so the
-     * refinement extrapolates only over FP64 residuals */
+    /* 1 (default): drop the DIIS history when the contractions switch to
+     * FP64, so the refinement extrapolates only over FP64 residuals (residuals
+     * of the reduced-precision phase carry errors near 1e-7 that slow the
+     * tail).  No effect on an FP64-only solve. */
     int diis_reset_on_refine;
     /* > 0: with a BF16x3 bulk phase, run FP32 contractions while
      * fp32_stage_below <= conv < refine_below, and FP64 only below it
