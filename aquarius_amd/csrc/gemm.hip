@@ -20,8 +20,10 @@
 // leading dimension padded to a multiple of 8 (16-byte vector loads).
 //
 // Tile order: the 1-D block id is remapped so that the blocks that land on
-// one XCD (ids are dealt round-robin over the 8 XCDs) cover consecutive tiles
-// with n fastest, i.e. they share A row-panels in that XCD's L2.
+// one XCD (ids are dealt round-robin over the 8 XCDs) cover consecutive tiles,
+// and consecutive tiles walk groups of 8 tile rows with m fastest, so one
+// XCD's co-resident blocks cover a square of tiles and share row panels of A
+// and column panels of B in that XCD's L2.
 #include "internal.hpp"
 
 #include <algorithm>
@@ -53,7 +55,26 @@ struct GemmArgs {
     int batch;
     int transA, transB;
     int off32;   // every per-thread tile offset of A and B fits 32 bits (FP64/FP32 interior fast path)
+    int raster;  // tile order: 0 = rows of tiles; G > 0 = groups of G tile rows (tile_of)
 };
+
+// tile t -> (tile row, tile col).  G == 0: rows of tiles, n fastest.  G > 0:
+// groups of G tile rows, m fastest within a group, so a run of consecutive
+// tiles (one XCD's co-resident blocks after remap_tile) covers a G x (run/G)
+// square of tiles and shares G row panels of A and run/G column panels of B.
+__device__ __forceinline__ void tile_of(int64_t t, int64_t tilesM, int64_t tilesN, int G, int64_t& tm, int64_t& tn) {
+    if (G <= 0) {
+        tm = t / tilesN;
+        tn = t % tilesN;
+        return;
+    }
+    const int64_t per = int64_t(G) * tilesN;
+    const int64_t first = (t / per) * G;
+    const int64_t gsz = min(int64_t(G), tilesM - first);
+    const int64_t r = t % per;
+    tm = first + r % gsz;
+    tn = r / gsz;
+}
 
 __device__ __forceinline__ int64_t remap_tile(int64_t bid, int64_t nb) {
     const int64_t full = (nb / 8) * 8;
@@ -224,7 +245,9 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
     const int64_t tilesN = (g.N + BN - 1) / BN;
     const int64_t tilesM = (g.M + BM - 1) / BM;
     const int64_t t = remap_tile(blockIdx.x, tilesM * tilesN);
-    const int64_t m0 = (t / tilesN) * BM, n0 = (t % tilesN) * BN;
+    int64_t tmi, tni;
+    tile_of(t, tilesM, tilesN, g.raster, tmi, tni);
+    const int64_t m0 = tmi * BM, n0 = tni * BN;
     const int bz = blockIdx.z;
     const int b = bz / g.nsplit, split = bz % g.nsplit;
     const int64_t kbeg = int64_t(split) * g.kchunk;
@@ -372,7 +395,9 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
     const int64_t tilesN = (g.N + BN - 1) / BN;
     const int64_t tilesM = (g.M + BM - 1) / BM;
     const int64_t t = remap_tile(blockIdx.x, tilesM * tilesN);
-    const int64_t m0 = (t / tilesN) * BM, n0 = (t % tilesN) * BN;
+    int64_t tmi, tni;
+    tile_of(t, tilesM, tilesN, g.raster, tmi, tni);
+    const int64_t m0 = tmi * BM, n0 = tni * BN;
     const int bz = blockIdx.z;
     const int b = bz / g.nsplit, split = bz % g.nsplit;
     const int64_t kbeg = int64_t(split) * g.kchunk;
@@ -717,6 +742,9 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     a.P = nullptr;
     // an element of a tile lies below 144 * ld + 256 elements from the tile origin
     a.off32 = (144 * std::max(d.lda, d.ldb) + 256) * 8 < (int64_t(1) << 31) ? 1 : 0;
+    // groups of 8 tile rows: measured on the 13824^3 ring GEMMs, BF16x3 279 -> 357 TFLOP/s
+    // (FP64 unchanged in time, MFMA-bound, with less L2-to-fabric traffic)
+    a.raster = 8;
 
     int variant = ctx->gemm_variant[d.fmt];
     if (variant < 0) variant = ctx->autotune ? tuned_variant(ctx, d, BM, BN, a) : default_gemm_variant(d.fmt);

@@ -35,8 +35,8 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 # of `bench.py --only fp64 --steps 1 --warmup 0`, scaled by the same counters on a plain 1 GiB copy
 # (fetch x1.9999, write x1.0); mean over the class's launches in one iteration.  Keyed by
 # (timing class, o, v); the per-launch table and the commands are in the profile summary
-# gemm_f64_128x128_o48v288_hbm_traffic.txt.
-TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NN", 48, 288): 107.7e9}
+# gemm_f64_128x128_NT_o48v288_hbm_traffic.txt.
+TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 87.7e9}
 
 
 def flops_per_iter(o, v):
