@@ -32,8 +32,8 @@ METRIC = "CCSD iter/s mixed vs FP64 at |ΔE|≤1e-6 Eh; served jobs/h + p99 time
 PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 
 # HBM bytes per launch of a kernel class, from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE)
-# of `bench.py --only fp64 --steps 1 --warmup 0`, scaled by the same counters on a plain 1 GiB copy
-# (fetch x1.9999, write x1.0); mean over the class's launches in one iteration.  Keyed by
+# of `bench.py --only fp64 --steps 1 --warmup 1 --autotune 0`, scaled by the same counters on a plain 1 GiB copy
+# (fetch x1.9999, write x1.0); mean over the class's launches in the timed iteration.  Keyed by
 # (timing class, o, v); the per-launch table and the commands are in the profile summary
 # gemm_f64_128x128_NT_o48v288_hbm_traffic.txt.
 TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 87.7e9}
