@@ -40,7 +40,7 @@ class RHFCCSDConfig(C.Structure):
 class TuneEntry(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("trans_a", C.c_int), ("trans_b", C.c_int),
                 ("m_bucket", C.c_int), ("n_bucket", C.c_int), ("k_bucket", C.c_int), ("batch_bucket", C.c_int),
-                ("variant", C.c_int), ("ms", C.c_float * 4)]
+                ("variant", C.c_int), ("ms", C.c_float * 5)]
 
 
 ITER_CB = C.CFUNCTYPE(None, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_void_p)

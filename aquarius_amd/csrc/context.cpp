@@ -282,7 +282,7 @@ int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant) {
     return guarded([&] {
         AQ_CHECK(ctx, AQ_ERR_INVALID, "null context");
         AQ_CHECK(precision >= 0 && precision <= 2, AQ_ERR_INVALID, "unknown precision");
-        AQ_CHECK(variant >= -1 && variant <= (precision == AQ_PREC_BF16X3 ? 3 : 2), AQ_ERR_INVALID,
+        AQ_CHECK(variant >= -1 && variant <= (precision == AQ_PREC_BF16X3 ? 4 : 2), AQ_ERR_INVALID,
                  "unknown GEMM variant");
         // AQ_PREC_* and the internal format ids coincide: FP64 = 0, FP32 = 1, BF16x3 = 2
         ctx->gemm_variant[precision] = variant;

@@ -473,20 +473,28 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
             xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][swz(nn, kq)]));
             xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][swz(nn, kq)]));
         }
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
+        auto load_a = [&](int i) {
             const int mm = wm * WTM + i * 16 + lane % 16;
             yh[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][swz(mm, kq)]));
             yl[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][swz(mm, kq)]));
+        };
+        // a 128 x 128 wave tile holds its accumulators in all 256 AGPRs:
+        // read each A fragment pair just before its row of blocks
+        constexpr bool lazy_a = FM * FN >= 64;
+        if (!lazy_a) {
+#pragma unroll
+            for (int i = 0; i < FM; ++i) load_a(i);
         }
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+        for (int i = 0; i < FM; ++i) {
+            if (lazy_a) load_a(i);
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i], acc[i][j], 0, 0, 0);
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i], acc[i][j], 0, 0, 0);
             }
+        }
     };
 
     const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
@@ -589,7 +597,8 @@ static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const Ge
         else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
         else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
     } else {
-        if (BM == 128 && BN == 128 && variant == 3) run_cfg_bf16<128, 128, 2, 2, 2>(ctx, a, grid);
+        if (BM == 256 && BN == 256) run_cfg_bf16<256, 256, 2, 2>(ctx, a, grid);
+        else if (BM == 128 && BN == 128 && variant == 3) run_cfg_bf16<128, 128, 2, 2, 2>(ctx, a, grid);
         else if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
         else if (BM == 256) run_cfg_bf16<256, 128, 2, 2>(ctx, a, grid);
         else if (BN == 256) run_cfg_bf16<128, 256, 2, 2>(ctx, a, grid);
@@ -601,11 +610,14 @@ static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const Ge
 
 // Block shape of a variant: BF16x3 variants 1 and 2 enlarge the 128 x 128
 // block to 256 x 128 / 128 x 256 (128 x 64 / 64 x 128 per wave), which reads a
-// third less LDS per MFMA; the FP64/FP32 variants change BK and occupancy only.
+// third less LDS per MFMA, and variant 4 to 256 x 256 (128 x 128 per wave, the
+// accumulators in all 256 AGPRs, one wave per SIMD); the FP64/FP32 variants
+// change BK and occupancy only.
 static void variant_shape(Fmt fmt, int variant, int64_t M, int64_t N, int& BM, int& BN) {
     if (fmt == FMT_BF16X2 && BM == 128 && BN == 128) {
         if (variant == 1 && M >= 256) BM = 256;
         else if (variant == 2 && N >= 256) BN = 256;
+        else if (variant == 4 && M >= 256 && N >= 256) { BM = 256; BN = 256; }
     }
 }
 
@@ -629,7 +641,7 @@ struct TuneKey {
 
 struct TuneEntry {
     int variant = -1;
-    float ms[4] = {-1.f, -1.f, -1.f, -1.f};
+    float ms[5] = {-1.f, -1.f, -1.f, -1.f, -1.f};
 };
 
 std::mutex g_tune_mu;
@@ -668,11 +680,11 @@ static int tuned_variant(aq_ctx* ctx, const GemmDesc& d, int BM, int BN, const G
         AQ_HIP(hipEventCreate(&e0));
         AQ_HIP(hipEventCreate(&e1));
         float best = 0.f;
-        const int nvar = d.fmt == FMT_BF16X2 ? 4 : 3;
+        const int nvar = d.fmt == FMT_BF16X2 ? 5 : 3;
         for (int v = 0; v < nvar; ++v) {
             int bm = BM, bn = BN;
             variant_shape(d.fmt, v, d.M, d.N, bm, bn);
-            if (d.fmt == FMT_BF16X2 && (v == 1 || v == 2) && bm == BM && bn == BN) continue;   // = variant 0
+            if (d.fmt == FMT_BF16X2 && (v == 1 || v == 2 || v == 4) && bm == BM && bn == BN) continue;   // = variant 0
             const int64_t tiles = ((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn);
             dim3 grid(unsigned(tiles), 1, unsigned(a.batch * a.nsplit));
             dispatch(ctx, d.fmt, bm, bn, v, t, grid);           // warm
@@ -789,7 +801,7 @@ int autotune_entries(aq_tune_entry* out, int cap) {
             e.k_bucket = kv.first.k;
             e.batch_bucket = kv.first.batch;
             e.variant = kv.second.variant;
-            for (int v = 0; v < 4; ++v) e.ms[v] = kv.second.ms[v];
+            for (int v = 0; v < 5; ++v) e.ms[v] = kv.second.ms[v];
         }
         ++n;
     }

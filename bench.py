@@ -545,7 +545,7 @@ def main():
     tab = aq.autotune_table()
     res["gemm_autotune"] = {"buckets": len(tab),
                             "variant_counts": {p: {str(v): sum(1 for e in tab if e["precision"] == c and e["variant"] == v)
-                                                   for v in (0, 1, 2, 3)}
+                                                   for v in (0, 1, 2, 3, 4)}
                                                for p, c in aq.PRECISIONS.items()}}
     ints.free()
     ctx.close()

@@ -78,8 +78,8 @@ int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
 /* Kernel variant of the large-tile GEMM for one arithmetic (AQ_PREC_*):
  * -1 = library default (or the autotuner's choice), FP64/FP32: 0 = BK 16,
  * 1 = BK 32, 2 = BK 16 with two waves per SIMD; BF16x3: 0 = 128x128 blocks,
- * 1 = 256x128, 2 = 128x256, 3 = 128x128 loading two k-tiles ahead.  A tuning knob: every variant
- * computes identical results. */
+ * 1 = 256x128, 2 = 128x256, 3 = 128x128 loading two k-tiles ahead, 4 = 256x256.
+ * A tuning knob: every variant computes identical results. */
 int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
 /* GEMM autotuning (no reference counterpart).  With enable != 0 the first
  * GEMM of every shape bucket (arithmetic, operand layouts, and M, N, K and
@@ -93,7 +93,7 @@ typedef struct aq_tune_entry {
     int device, precision, trans_a, trans_b;
     int m_bucket, n_bucket, k_bucket, batch_bucket;   /* round(2*log2(x)) */
     int variant;                                      /* chosen */
-    float ms[4];                                      /* trial time per variant, -1 = not tried */
+    float ms[5];                                      /* trial time per variant, -1 = not tried */
 } aq_tune_entry;
 /* Copy up to `cap` entries of the tuning table; *n receives the total. */
 int aq_autotune_table(aq_tune_entry* out, int cap, int* n);

@@ -116,16 +116,16 @@ def test_contract_reduced_precision(ctx, prec, tol, ia, ib, ic, o, v):
     check_contract(ctx, ia, ib, ic, dims_for(o, v), beta=0.0, seed=21, tol=tol, prec=prec)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("ia,ib,ic,dim", [
     ("ak", "kb", "ab", {"a": 300, "b": 270, "k": 200}),
     ("amie", "ebmj", "abij", {"a": 40, "b": 40, "e": 40, "m": 8, "i": 8, "j": 8}),
     ("ak", "kb", "ab", {"a": 520, "b": 390, "k": 1100}),
 ])
 def test_bf16x3_tile_variants(ctx, variant, ia, ib, ic, dim):
-    """Every BF16x3 kernel (128x128, 256x128, 128x256 blocks; 128x128 loading
-    two k-tiles ahead) on ragged edges, odd and even k-tile counts, with and
-    without split-K."""
+    """Every BF16x3 kernel (128x128, 256x128, 128x256, 256x256 blocks; 128x128
+    loading two k-tiles ahead) on ragged edges, odd and even k-tile counts,
+    with and without split-K."""
     print(f"case: bf16x3 variant {variant} {ia},{ib}->{ic} {dim}", flush=True)
     ctx.set_gemm_variant("bf16x3", variant)
     try:
@@ -147,7 +147,7 @@ def test_gemm_variants_and_autotuner_are_bitwise_identical(ctx, prec):
         outs = []
         ctx.set_precision(prec, 0.0)
         try:
-            for v in ((0, 1, 2, 3, -1) if prec == "bf16x3" else (0, 1, 2, -1)):
+            for v in ((0, 1, 2, 3, 4, -1) if prec == "bf16x3" else (0, 1, 2, -1)):
                 ctx.set_gemm_variant(prec, v)
                 ctx.set_autotune(v == -1)
                 Cc = ctx.tensor((dim["a"], dim["b"]), zero=True)
@@ -165,7 +165,7 @@ def test_gemm_variants_and_autotuner_are_bitwise_identical(ctx, prec):
     table = aq.autotune_table()
     code = aq.PRECISIONS[prec]
     mine = [e for e in table if e["precision"] == code and e["m_bucket"] == 22 and e["n_bucket"] == 22]
-    assert mine and mine[0]["variant"] in (0, 1, 2, 3)
+    assert mine and mine[0]["variant"] in (0, 1, 2, 3, 4)
     assert mine[0]["ms"][mine[0]["variant"]] == min(m for m in mine[0]["ms"] if m >= 0)
 
 
