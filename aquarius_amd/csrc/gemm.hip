@@ -730,14 +730,26 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     constexpr int64_t KSTEP = 32;
     const int64_t base_tiles = ((d.M + BM - 1) / BM) * ((d.N + BN - 1) / BN);
     AQ_CHECK(base_tiles < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "GEMM grid too large");
+    // The split count s (at most ~two rounds of W = 512 co-resident blocks,
+    // 8 k-steps per split, 256 splits) minimises the rounds per unit of work,
+    // ceil(T*s / W) / s with T tiles: e.g. 324 tiles split 3 ways fill two
+    // rounds (972 blocks) where 4 ways need three (1296).  Ties take the
+    // smaller s (less partial traffic).
     int nsplit = 1;
     const int64_t ksteps = (d.K + KSTEP - 1) / KSTEP;
-    const int64_t want = 1024;
-    if (base_tiles * d.batch < 512 && ksteps >= 16) {
-        int64_t s = (want + base_tiles * d.batch - 1) / (base_tiles * d.batch);
-        s = std::min<int64_t>(s, ksteps / 8);
-        s = std::min<int64_t>(s, 256);
-        nsplit = int(std::max<int64_t>(s, 1));
+    const int64_t T = base_tiles * d.batch, W = 512;
+    if (T < W && ksteps >= 16) {
+        const int64_t smax = std::max<int64_t>(1, std::min<int64_t>({(2 * W + T - 1) / T, ksteps / 8, 256}));
+        int64_t best = 1;
+        double best_cost = 1.0;   // s = 1: one round per unit of work
+        for (int64_t sc = 2; sc <= smax; ++sc) {
+            const double cost = double((T * sc + W - 1) / W) / double(sc);
+            if (cost < best_cost - 1e-12) {
+                best_cost = cost;
+                best = sc;
+            }
+        }
+        nsplit = int(best);
         while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
     }
     int64_t kchunk = ((ksteps + nsplit - 1) / nsplit) * KSTEP;

@@ -126,3 +126,25 @@ def test_graph_serving_mix_equals_eager_alone():
     for a, m, job in zip(alone, mix, jobs):
         assert a["status"] == aq.JOB_DONE and m["status"] == aq.JOB_DONE, (job, a["error"], m["error"])
         assert a["energy"] == m["energy"] and a["iterations"] == m["iterations"], (job, a, m)
+
+
+def test_hbar_after_graph_replays_equals_eager():
+    """The Hbar products formed after a solve that replayed graphs read the
+    operand cache the graph left behind; they must equal the eager solve's."""
+    print("case: FP64 o=5 v=20, 10 iterations then hbar(with_wabej)", flush=True)
+    out = []
+    for graph in (0, 1):
+        with aq.Context() as ctx:
+            ints = aq.Integrals.synthetic(ctx, 45, 5, 20)
+            cc = aq.RHFCCSD(ctx, ints, graph=graph)
+            for _ in range(10):
+                cc.iterate()
+            cc.hbar(with_wabej=True)
+            out.append({k: cc.tensor(k).numpy() for k in ("FAE", "FMI", "FME", "WMNIJ", "WMNEJ", "WAMIJ",
+                                                          "WAMEI", "WAMIE", "WABEJ")})
+            if graph:
+                assert cc.graph_stats()[1] == 8
+            cc.close()
+            ints.free()
+    for k in out[0]:
+        assert np.array_equal(out[0][k], out[1][k]), k
