@@ -261,7 +261,11 @@ struct aq_rhfccsd {
     bool capture_stage(int prec) {
         CaptureLog log;
         const uint64_t gen0 = ctx->mem_gen;
-        AQ_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();   // e.g. the caller is capturing this stream already
+            graph.failed += 1;
+            return false;
+        }
         ctx->capture = &log;
         bool ok = true;
         try {
