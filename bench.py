@@ -16,7 +16,7 @@ explicit K times exactly K iterations (the solve then continues untimed to
 convergence so its energy can be compared with the FP64 one).
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this engine runs on one
-GPU (see DESIGN.md for how it would shard).
+GPU (sharding over GPUs would split Z2's output pair index; it is not built).
 """
 
 import argparse
@@ -376,8 +376,8 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"metric": METRIC, "error": "this engine runs on one GPU; multi-GPU sharding is a design "
-                          "(DESIGN.md), not a launchable path", "n_gpus": a.gpus}))
+        print(json.dumps({"metric": METRIC, "error": "this engine runs on one GPU; sharding over several GPUs is "
+                          "not built", "n_gpus": a.gpus}))
         sys.exit(2)
 
     if a.seed is None:
