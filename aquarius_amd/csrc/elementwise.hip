@@ -662,6 +662,63 @@ double pair_asymmetry(aq_ctx* ctx, const double* X, int n1, int n2, int slot) {
     return sync_result(ctx, slot);
 }
 
+// ------------------------------------------------------------ AO integrals
+//
+// (pq|rs) [n,n,n,n] (p fastest) from the compact canonical shell-quartet
+// storage (fe::CompactEri): the shell pair of (p,q) and of (r,s) is ordered
+// larger shell first, the pair with the larger index is the bra, and the value
+// is element [fA][fB][fC][fD] of block (bra, ket).  Indices at or past `total`
+// read as zero (the layout never produces them; the guard keeps a corrupted
+// table from reading out of bounds).
+struct EriTables {
+    const int* shell_of;
+    const int* fn_of;
+    const int* nf;
+    const int64_t* pair_size;
+    const int64_t* prefix;
+    const int64_t* rowstart;
+};
+
+__global__ __launch_bounds__(TPB) void k_eri_expand(const double* __restrict__ compact, EriTables t, int64_t n,
+                                                    int64_t total, double* __restrict__ out) {
+    const int64_t n4 = n * n * n * n;
+    for (int64_t k = int64_t(blockIdx.x) * TPB + threadIdx.x; k < n4; k += int64_t(gridDim.x) * TPB) {
+        int64_t x = k;
+        const int64_t p = x % n; x /= n;
+        const int64_t q = x % n; x /= n;
+        const int64_t r = x % n, s = x / n;
+        int A = t.shell_of[p], fA = t.fn_of[p], B = t.shell_of[q], fB = t.fn_of[q];
+        int Cs = t.shell_of[r], fC = t.fn_of[r], D = t.shell_of[s], fD = t.fn_of[s];
+        if (A < B) { int u = A; A = B; B = u; u = fA; fA = fB; fB = u; }
+        if (Cs < D) { int u = Cs; Cs = D; D = u; u = fC; fC = fD; fD = u; }
+        // within a pair of one shell and between bra and ket of one pair, the
+        // larger function index first: all eight permutations of (p,q,r,s)
+        // read the same stored value, so the tensor is exactly symmetric
+        if (A == B && fA < fB) { int u = fA; fA = fB; fB = u; }
+        if (Cs == D && fC < fD) { int u = fC; fC = fD; fD = u; }
+        int64_t P1 = int64_t(A) * (A + 1) / 2 + B, P2 = int64_t(Cs) * (Cs + 1) / 2 + D;
+        if (P1 < P2 || (P1 == P2 && (fA < fC || (fA == fC && fB < fD)))) {
+            int u = A; A = Cs; Cs = u; u = B; B = D; D = u;
+            u = fA; fA = fC; fC = u; u = fB; fB = fD; fD = u;
+            const int64_t w = P1; P1 = P2; P2 = w;
+        }
+        const int64_t idx = t.rowstart[P1] + t.pair_size[P1] * t.prefix[P2] +
+                            ((int64_t(fA) * t.nf[B] + fB) * t.nf[Cs] + fC) * t.nf[D] + fD;
+        out[k] = idx < total ? compact[idx] : 0.0;
+    }
+}
+
+void launch_eri_expand(aq_ctx* ctx, const double* compact, const int* shell_of, const int* fn_of, const int* nf,
+                       const int64_t* pair_size, const int64_t* prefix, const int64_t* rowstart, int64_t n,
+                       int64_t total, double* out) {
+    const int64_t n4 = n * n * n * n;
+    if (n4 == 0) return;
+    TimedLaunch tl(ctx, "eri_expand", 0.0, 16.0 * double(n4));
+    EriTables t{shell_of, fn_of, nf, pair_size, prefix, rowstart};
+    hipLaunchKernelGGL(k_eri_expand, dim3(grid_for(n4, TPB * 4)), dim3(TPB), 0, ctx->stream, compact, t, n, total, out);
+    AQ_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ DIIS
 
 constexpr int MAXK = 16;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <array>
+#include <cstdint>
 #include <map>
 #include <string>
 #include <vector>
@@ -60,6 +61,18 @@ void boys(int mmax, double T, double* F);
 // S, T, V: nao x nao (symmetric).  eri[m + n N + l N^2 + s N^3] = (mn|ls).
 void ints_1e(const MoleculeData& m, double* S, double* T, double* V);
 void ints_2e(const MoleculeData& m, int nthreads, double* eri);
+
+// Compact storage of the canonical shell quartets: bra pair i = (a >= b)
+// index a(a+1)/2 + b, ket pair j <= i; block (i, j) of pair_size[i] *
+// pair_size[j] values [fa][fb][fc][fd] starts at rowstart[i] + pair_size[i] *
+// prefix[j].  shell_of / fn_of map an AO to its shell and its function within.
+struct CompactEri {
+    std::vector<int64_t> pair_size, prefix, rowstart;
+    std::vector<int> shell_of, fn_of, nf;
+    int64_t total = 0;
+};
+CompactEri compact_layout(const MoleculeData& m);
+void ints_2e_compact(const MoleculeData& m, int nthreads, const CompactEri& c, double* out);
 
 // Symmetric eigenproblem A = U diag(w) U^T (cyclic Jacobi), n x n, A overwritten;
 // eigenvalues ascending, U column-major (column k = eigenvector k).

@@ -238,6 +238,12 @@ void launch_ladder_unpack(aq_ctx* ctx, const double* Sp, const double* Sm, int v
 // max |X(p,q,r,s) - X(q,p,s,r)| of X[n1,n1,n2,n2] (synchronising)
 double pair_asymmetry(aq_ctx* ctx, const double* X, int n1, int n2, int slot);
 
+// AO integrals (elementwise.hip): (pq|rs) [n,n,n,n] from the compact
+// canonical shell-quartet storage (fe::CompactEri), all pointers on the device
+void launch_eri_expand(aq_ctx* ctx, const double* compact, const int* shell_of, const int* fn_of, const int* nf,
+                       const int64_t* pair_size, const int64_t* prefix, const int64_t* rowstart, int64_t n,
+                       int64_t total, double* out);
+
 // DIIS helpers (elementwise.hip)
 void launch_multi_dot(aq_ctx* ctx, const double* x, const double* const* ys, int k, int64_t n,
                       int first_slot);

@@ -20,6 +20,7 @@
 #include <atomic>
 #include <exception>
 #include <cmath>
+#include <cstring>
 #include <mutex>
 #include <thread>
 
@@ -423,11 +424,17 @@ void transform_index(std::vector<double>& x, int dims[4], int k, const Shell& s,
 
 }  // namespace
 
-void ints_2e(const MoleculeData& m, int nthreads, double* eri) {
+// Every canonical shell quartet (bra pair i >= ket pair j, each pair a >= b)
+// evaluated once, its pure-harmonic block [fa][fb][fc][fd] handed to
+// sink(i, j, block) (block == nullptr: screened to zero).  Runs `nthreads`
+// workers over bra pairs in descending cost; distinct (i, j) are distinct calls.
+template <class Sink>
+static void for_each_quartet(const MoleculeData& m, int nthreads, const std::vector<PairData>*& out_pairs,
+                             std::vector<PairData>& pairs, Sink&& sink) {
     const int ns = int(m.shells.size());
-    const size_t N = size_t(m.nao);
     const size_t npair = size_t(ns) * size_t(ns + 1) / 2;
-    std::vector<PairData> pairs(npair);
+    pairs.assign(npair, PairData());
+    out_pairs = &pairs;
     if (nthreads <= 0) nthreads = int(std::max(1u, std::thread::hardware_concurrency()));
     nthreads = std::min(nthreads, 64);
 
@@ -485,46 +492,98 @@ void ints_2e(const MoleculeData& m, int nthreads, double* eri) {
                 const Shell& sc = m.shells[size_t(ket.a)];
                 const Shell& sd = m.shells[size_t(ket.b)];
                 int dims[4] = {ncart(sa.L), ncart(sb.L), ncart(sc.L), ncart(sd.L)};
-                const bool zero = bra.Q * ket.Q < cutoff;
-                if (!zero) {
-                    quartet(bra, ket, w);
-                    w.ao.swap(w.out);
-                    transform_index(w.ao, dims, 0, sa, tmp);
-                    transform_index(w.ao, dims, 1, sb, tmp);
-                    transform_index(w.ao, dims, 2, sc, tmp);
-                    transform_index(w.ao, dims, 3, sd, tmp);
-                } else {
-                    dims[0] = sa.nf;
-                    dims[1] = sb.nf;
-                    dims[2] = sc.nf;
-                    dims[3] = sd.nf;
+                if (bra.Q * ket.Q < cutoff) {
+                    sink(i, j, static_cast<const double*>(nullptr));
+                    continue;
                 }
-                for (int f0 = 0; f0 < dims[0]; ++f0)
-                    for (int f1 = 0; f1 < dims[1]; ++f1)
-                        for (int f2 = 0; f2 < dims[2]; ++f2)
-                            for (int f3 = 0; f3 < dims[3]; ++f3) {
-                                const double x =
-                                    zero ? 0.0
-                                         : w.ao[((size_t(f0) * size_t(dims[1]) + size_t(f1)) * size_t(dims[2]) + size_t(f2)) *
-                                                    size_t(dims[3]) +
-                                                size_t(f3)];
-                                const size_t M = size_t(sa.ao0 + f0), Nn = size_t(sb.ao0 + f1),
-                                             Lc = size_t(sc.ao0 + f2), S = size_t(sd.ao0 + f3);
-                                auto put = [&](size_t p, size_t q, size_t r2, size_t s2) {
-                                    eri[p + N * (q + N * (r2 + N * s2))] = x;
-                                };
-                                put(M, Nn, Lc, S);
-                                put(Nn, M, Lc, S);
-                                put(M, Nn, S, Lc);
-                                put(Nn, M, S, Lc);
-                                put(Lc, S, M, Nn);
-                                put(S, Lc, M, Nn);
-                                put(Lc, S, Nn, M);
-                                put(S, Lc, Nn, M);
-                            }
+                quartet(bra, ket, w);
+                w.ao.swap(w.out);
+                transform_index(w.ao, dims, 0, sa, tmp);
+                transform_index(w.ao, dims, 1, sb, tmp);
+                transform_index(w.ao, dims, 2, sc, tmp);
+                transform_index(w.ao, dims, 3, sd, tmp);
+                sink(i, j, static_cast<const double*>(w.ao.data()));
             }
         },
         npair);
+}
+
+void ints_2e(const MoleculeData& m, int nthreads, double* eri) {
+    const size_t N = size_t(m.nao);
+    std::vector<PairData> pairs;
+    const std::vector<PairData>* P = nullptr;
+    for_each_quartet(m, nthreads, P, pairs, [&](size_t i, size_t j, const double* blk) {
+        const Shell& sa = m.shells[size_t((*P)[i].a)];
+        const Shell& sb = m.shells[size_t((*P)[i].b)];
+        const Shell& sc = m.shells[size_t((*P)[j].a)];
+        const Shell& sd = m.shells[size_t((*P)[j].b)];
+        const int d0 = sa.nf, d1 = sb.nf, d2 = sc.nf, d3 = sd.nf;
+        for (int f0 = 0; f0 < d0; ++f0)
+            for (int f1 = 0; f1 < d1; ++f1)
+                for (int f2 = 0; f2 < d2; ++f2)
+                    for (int f3 = 0; f3 < d3; ++f3) {
+                        const double x =
+                            blk ? blk[((size_t(f0) * size_t(d1) + size_t(f1)) * size_t(d2) + size_t(f2)) * size_t(d3) +
+                                      size_t(f3)]
+                                : 0.0;
+                        const size_t M = size_t(sa.ao0 + f0), Nn = size_t(sb.ao0 + f1), Lc = size_t(sc.ao0 + f2),
+                                     S = size_t(sd.ao0 + f3);
+                        auto put = [&](size_t p, size_t q, size_t r2, size_t s2) {
+                            eri[p + N * (q + N * (r2 + N * s2))] = x;
+                        };
+                        put(M, Nn, Lc, S);
+                        put(Nn, M, Lc, S);
+                        put(M, Nn, S, Lc);
+                        put(Nn, M, S, Lc);
+                        put(Lc, S, M, Nn);
+                        put(S, Lc, M, Nn);
+                        put(Lc, S, Nn, M);
+                        put(S, Lc, Nn, M);
+                    }
+    });
+}
+
+CompactEri compact_layout(const MoleculeData& m) {
+    CompactEri c;
+    const int ns = int(m.shells.size());
+    const size_t npair = size_t(ns) * size_t(ns + 1) / 2;
+    c.pair_size.resize(npair);
+    c.prefix.resize(npair);
+    c.rowstart.resize(npair);
+    for (int a = 0, idx = 0; a < ns; ++a)
+        for (int b = 0; b <= a; ++b, ++idx)
+            c.pair_size[size_t(idx)] = int64_t(m.shells[size_t(a)].nf) * int64_t(m.shells[size_t(b)].nf);
+    int64_t pre = 0, row = 0;
+    for (size_t i = 0; i < npair; ++i) {
+        c.prefix[i] = pre;                       // sum of pair sizes before i
+        pre += c.pair_size[i];
+        c.rowstart[i] = row;                     // blocks (i', j' <= i') for i' < i
+        row += c.pair_size[i] * pre;             // pre now includes pair i
+    }
+    c.total = row;
+    AQ_CHECK(npair == 0 || c.rowstart[npair - 1] + c.pair_size[npair - 1] * (c.prefix[npair - 1] + c.pair_size[npair - 1]) == c.total,
+             AQ_ERR_INVALID, "inconsistent compact ERI layout");
+    c.shell_of.resize(size_t(m.nao));
+    c.fn_of.resize(size_t(m.nao));
+    for (int s = 0; s < ns; ++s)
+        for (int f = 0; f < m.shells[size_t(s)].nf; ++f) {
+            c.shell_of[size_t(m.shells[size_t(s)].ao0 + f)] = s;
+            c.fn_of[size_t(m.shells[size_t(s)].ao0 + f)] = f;
+        }
+    c.nf.resize(size_t(ns));
+    for (int s = 0; s < ns; ++s) c.nf[size_t(s)] = m.shells[size_t(s)].nf;
+    return c;
+}
+
+void ints_2e_compact(const MoleculeData& m, int nthreads, const CompactEri& c, double* out) {
+    std::vector<PairData> pairs;
+    const std::vector<PairData>* P = nullptr;
+    for_each_quartet(m, nthreads, P, pairs, [&](size_t i, size_t j, const double* blk) {
+        const size_t n = size_t(c.pair_size[i]) * size_t(c.pair_size[j]);
+        double* dst = out + c.rowstart[i] + c.pair_size[i] * c.prefix[j];
+        if (blk) std::memcpy(dst, blk, n * sizeof(double));
+        else std::memset(dst, 0, n * sizeof(double));
+    });
 }
 
 }  // namespace fe

@@ -189,15 +189,48 @@ int aq_ints_2e(const aq_molecule* mol, int nthreads, double* eri) {
     });
 }
 
+// The host evaluates each canonical shell quartet once into a compact pinned
+// buffer (about n^4/8 values); one copy moves it to the device, where a
+// gather kernel expands the 8-fold permutational symmetry into (mn|ls).
 int aq_ints_2e_device(aq_ctx* ctx, const aq_molecule* mol, int nthreads, aq_tensor** out) {
     return guarded([&] {
         AQ_CHECK(ctx && mol && out, AQ_ERR_INVALID, "null argument");
         const int64_t n = mol->d.nao;
-        std::vector<double> host(size_t(n) * size_t(n) * size_t(n) * size_t(n));
-        fe::ints_2e(mol->d, nthreads, host.data());
+        const fe::CompactEri lay = fe::compact_layout(mol->d);
+        struct Pinned {
+            double* p = nullptr;
+            ~Pinned() {
+                if (p) (void)hipHostFree(p);
+            }
+        } host;
+        AQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&host.p), size_t(std::max<int64_t>(lay.total, 1)) * 8, 0));
+        fe::ints_2e_compact(mol->d, nthreads, lay, host.p);
         Owned e(ctx, {n, n, n, n});
-        ck(aq_write_all(ctx, e.t, host.data()));
-        ck(aq_ctx_sync(ctx));
+        {
+            Scratch dc(ctx, size_t(std::max<int64_t>(lay.total, 1)) * 8);
+            Scratch dt(ctx, (lay.shell_of.size() + lay.fn_of.size() + lay.nf.size()) * sizeof(int) +
+                                (lay.pair_size.size() * 3) * sizeof(int64_t) + 64);
+            char* base = dt.as<char>();
+            int64_t* d_ps = reinterpret_cast<int64_t*>(base);
+            int64_t* d_pre = d_ps + lay.pair_size.size();
+            int64_t* d_row = d_pre + lay.prefix.size();
+            int* d_sh = reinterpret_cast<int*>(d_row + lay.rowstart.size());
+            int* d_fn = d_sh + lay.shell_of.size();
+            int* d_nf = d_fn + lay.fn_of.size();
+            auto up = [&](void* dst, const void* src, size_t bytes) {
+                if (bytes) AQ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+            };
+            up(dc.p, host.p, size_t(lay.total) * 8);
+            up(d_ps, lay.pair_size.data(), lay.pair_size.size() * sizeof(int64_t));
+            up(d_pre, lay.prefix.data(), lay.prefix.size() * sizeof(int64_t));
+            up(d_row, lay.rowstart.data(), lay.rowstart.size() * sizeof(int64_t));
+            up(d_sh, lay.shell_of.data(), lay.shell_of.size() * sizeof(int));
+            up(d_fn, lay.fn_of.data(), lay.fn_of.size() * sizeof(int));
+            up(d_nf, lay.nf.data(), lay.nf.size() * sizeof(int));
+            launch_eri_expand(ctx, dc.as<double>(), d_sh, d_fn, d_nf, d_ps, d_pre, d_row, n, lay.total, e.t->data);
+            AQ_HIP(hipStreamSynchronize(ctx->stream));   // the host tables and pinned buffer die here
+        }
+        e.t->version += 1;
         *out = e.release();
     });
 }

@@ -67,3 +67,59 @@ def test_water_cluster_construction():
         np.testing.assert_allclose(np.linalg.norm(x[:, None] - x[None], axis=-1), ref, atol=1e-12)
     m = aq.Molecule(a, "cc-pVDZ")
     assert (m.nao, m.nelectrons) == (192, 80)
+
+
+def _compact_eri_index(nf):
+    """The device expansion's index map (k_eri_expand over fe::CompactEri),
+    restated: shells with nf[s] functions, pairs a >= b indexed a(a+1)/2 + b,
+    block (i, j <= i) at rowstart[i] + size[i] * prefix[j]."""
+    shell_of, fn_of = [], []
+    for s, k in enumerate(nf):
+        for f in range(k):
+            shell_of.append(s)
+            fn_of.append(f)
+    ns = len(nf)
+    size = [nf[a] * nf[b] for a in range(ns) for b in range(a + 1)]
+    prefix, rowstart, pre, row = [], [], 0, 0
+    for sz in size:
+        prefix.append(pre)
+        pre += sz
+        rowstart.append(row)
+        row += sz * pre
+
+    def index(p, q, r, s):
+        A, fA, B, fB = shell_of[p], fn_of[p], shell_of[q], fn_of[q]
+        C, fC, D, fD = shell_of[r], fn_of[r], shell_of[s], fn_of[s]
+        if A < B:
+            A, B, fA, fB = B, A, fB, fA
+        if C < D:
+            C, D, fC, fD = D, C, fD, fC
+        if A == B and fA < fB:
+            fA, fB = fB, fA
+        if C == D and fC < fD:
+            fC, fD = fD, fC
+        P1, P2 = A * (A + 1) // 2 + B, C * (C + 1) // 2 + D
+        if P1 < P2 or (P1 == P2 and (fA < fC or (fA == fC and fB < fD))):
+            A, B, C, D, fA, fB, fC, fD, P1, P2 = C, D, A, B, fC, fD, fA, fB, P2, P1
+        return rowstart[P1] + size[P1] * prefix[P2] + ((fA * nf[B] + fB) * nf[C] + fC) * nf[D] + fD
+
+    return len(shell_of), row, index
+
+
+def test_compact_eri_index_is_in_bounds_and_symmetric():
+    """Every (p,q,r,s) maps inside the compact buffer, and its eight
+    permutations map to the same stored value (exact 8-fold symmetry)."""
+    import itertools
+    n, total, index = _compact_eri_index([1, 1, 3, 1, 3, 5, 1, 3])
+    used = set()
+    for p, q, r, s in itertools.product(range(n), repeat=4):
+        k = index(p, q, r, s)
+        assert 0 <= k < total
+        for pp in [(q, p, r, s), (p, q, s, r), (q, p, s, r), (r, s, p, q), (s, r, p, q), (r, s, q, p),
+                   (s, r, q, p)]:
+            assert index(*pp) == k
+        used.add(k)
+    # every unique (pq|rs) class has its own slot
+    classes = {min(((p, q, r, s), (q, p, r, s), (p, q, s, r), (q, p, s, r), (r, s, p, q), (s, r, p, q),
+                    (r, s, q, p), (s, r, q, p))) for p, q, r, s in itertools.product(range(n), repeat=4)}
+    assert len(used) == len(classes)
