@@ -15,6 +15,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <numeric>
@@ -32,56 +33,131 @@ using aqamd::fe::solve;
 namespace aqamd {
 namespace fe {
 
-// Cyclic Jacobi (Rutishauser's rotation: theta = (a_qq - a_pp) / (2 a_pq),
-// t = sgn(theta) / (|theta| + sqrt(theta^2 + 1))).  A is n x n symmetric
-// row-major; U receives the eigenvectors as columns of a row-major matrix.
+// Symmetric eigensolver: Householder reduction to tridiagonal form, then the
+// implicit QL iteration with Wilkinson shifts (Bowdler, Martin, Reinsch,
+// Wilkinson, Numer. Math. 11, 293 (1968)).  The orthogonal factor is kept as
+// Q^T (row k = k-th column of Q), so both the reflections and the plane
+// rotations update contiguous rows.  A is n x n symmetric row-major (destroyed);
+// w receives the eigenvalues ascending, U the eigenvectors as the columns of a
+// row-major matrix.
 void eigh(int n, std::vector<double>& A, std::vector<double>& w, std::vector<double>& U) {
     const size_t N = size_t(n);
-    U.assign(N * N, 0.0);
-    for (size_t i = 0; i < N; ++i) U[i * N + i] = 1.0;
+    std::vector<double> QT(N * N, 0.0);
+    for (size_t i = 0; i < N; ++i) QT[i * N + i] = 1.0;
     auto a = [&](size_t i, size_t j) -> double& { return A[i * N + j]; };
-    double scale = 0.0;
-    for (size_t i = 0; i < N * N; ++i) scale += A[i] * A[i];
-    for (int sweep = 0; sweep < 100; ++sweep) {
-        double off = 0.0;
-        for (size_t p = 0; p < N; ++p)
-            for (size_t q = p + 1; q < N; ++q) off += a(p, q) * a(p, q);
-        if (off <= 1e-32 * scale || off == 0.0) break;
-        for (size_t p = 0; p < N; ++p)
-            for (size_t q = p + 1; q < N; ++q) {
-                const double apq = a(p, q);
-                if (std::fabs(apq) < 1e-300) continue;
-                const double theta = (a(q, q) - a(p, p)) / (2.0 * apq);
-                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-                for (size_t k = 0; k < N; ++k) {
-                    const double akp = a(k, p), akq = a(k, q);
-                    a(k, p) = c * akp - s * akq;
-                    a(k, q) = s * akp + c * akq;
-                }
-                for (size_t k = 0; k < N; ++k) {
-                    const double apk = a(p, k), aqk = a(q, k);
-                    a(p, k) = c * apk - s * aqk;
-                    a(q, k) = s * apk + c * aqk;
-                }
-                a(p, q) = a(q, p) = 0.0;
-                for (size_t k = 0; k < N; ++k) {
-                    const double ukp = U[k * N + p], ukq = U[k * N + q];
-                    U[k * N + p] = c * ukp - s * ukq;
-                    U[k * N + q] = s * ukp + c * ukq;
-                }
-            }
+    std::vector<double> v(N), pv(N), u(N);
+
+    // A <- H A H with H = I - beta v v^T acting on rows/columns k+1..n-1
+    for (size_t k = 0; k + 2 < N; ++k) {
+        const size_t m = N - k - 1;   // length of the reflected part
+        double xn = 0.0;
+        for (size_t t = 0; t < m; ++t) xn += a(k + 1 + t, k) * a(k + 1 + t, k);
+        xn = std::sqrt(xn);
+        if (xn == 0.0) continue;
+        const double alpha = a(k + 1, k) > 0.0 ? -xn : xn;
+        for (size_t t = 0; t < m; ++t) v[t] = a(k + 1 + t, k);
+        v[0] -= alpha;
+        double vv = 0.0;
+        for (size_t t = 0; t < m; ++t) vv += v[t] * v[t];
+        if (vv == 0.0) continue;
+        const double beta = 2.0 / vv;
+        // p = beta A_sub v, K = beta (v.p) / 2, q = p - K v, A_sub -= v q^T + q v^T
+        double vp = 0.0;
+        for (size_t r = 0; r < m; ++r) {
+            const double* row = &A[(k + 1 + r) * N + k + 1];
+            double acc = 0.0;
+            for (size_t t = 0; t < m; ++t) acc += row[t] * v[t];
+            pv[r] = beta * acc;
+            vp += v[r] * pv[r];
+        }
+        const double K = 0.5 * beta * vp;
+        for (size_t r = 0; r < m; ++r) pv[r] -= K * v[r];
+        for (size_t r = 0; r < m; ++r) {
+            double* row = &A[(k + 1 + r) * N + k + 1];
+            const double vr = v[r], qr = pv[r];
+            for (size_t t = 0; t < m; ++t) row[t] -= vr * pv[t] + qr * v[t];
+        }
+        a(k + 1, k) = a(k, k + 1) = alpha;
+        for (size_t t = 1; t < m; ++t) a(k + 1 + t, k) = a(k, k + 1 + t) = 0.0;
+        // Q <- Q H, i.e. rows k+1.. of Q^T: QT_sub -= beta v (v^T QT_sub)
+        std::fill(u.begin(), u.end(), 0.0);
+        for (size_t t = 0; t < m; ++t) {
+            const double* row = &QT[(k + 1 + t) * N];
+            const double vt = v[t];
+            for (size_t c = 0; c < N; ++c) u[c] += vt * row[c];
+        }
+        for (size_t t = 0; t < m; ++t) {
+            double* row = &QT[(k + 1 + t) * N];
+            const double f = beta * v[t];
+            for (size_t c = 0; c < N; ++c) row[c] -= f * u[c];
+        }
     }
+
+    // implicit QL on the tridiagonal (d, e): e[i] couples i and i+1
+    std::vector<double> d(N), e(N, 0.0);
+    for (size_t i = 0; i < N; ++i) d[i] = a(i, i);
+    for (size_t i = 0; i + 1 < N; ++i) e[i] = a(i + 1, i);
+    const double eps = std::numeric_limits<double>::epsilon();
+    auto rotate = [&](size_t i, double c, double sn) {   // rows i, i+1 of Q^T
+        double* r0 = &QT[i * N];
+        double* r1 = &QT[(i + 1) * N];
+        for (size_t c2 = 0; c2 < N; ++c2) {
+            const double f = r1[c2];
+            r1[c2] = sn * r0[c2] + c * f;
+            r0[c2] = c * r0[c2] - sn * f;
+        }
+    };
+    for (size_t l = 0; l < N; ++l) {
+        int iter = 0;
+        for (;;) {
+            size_t m = l;
+            for (; m + 1 < N; ++m) {
+                const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+                if (std::fabs(e[m]) <= eps * dd) break;
+            }
+            if (m == l) break;
+            AQ_CHECK(++iter <= 60, AQ_ERR_INVALID, "eigensolver did not converge");
+            double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+            double r = std::hypot(g, 1.0);
+            g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+            double sn = 1.0, c = 1.0, p = 0.0;
+            bool early = false;
+            for (size_t i = m; i-- > l;) {
+                const double f = sn * e[i], b = c * e[i];
+                r = std::hypot(f, g);
+                e[i + 1] = r;
+                if (r == 0.0) {   // split: the rest decouples
+                    d[i + 1] -= p;
+                    e[m] = 0.0;
+                    early = true;
+                    break;
+                }
+                sn = f / r;
+                c = g / r;
+                g = d[i + 1] - p;
+                r = (d[i] - g) * sn + 2.0 * c * b;
+                p = sn * r;
+                d[i + 1] = g + p;
+                g = c * r - b;
+                rotate(i, c, sn);
+            }
+            if (early) continue;
+            d[l] -= p;
+            e[l] = g;
+            e[m] = 0.0;
+        }
+    }
+
     std::vector<size_t> order(N);
     std::iota(order.begin(), order.end(), size_t(0));
-    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return a(x, x) < a(y, y); });
+    std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return d[x] < d[y]; });
     w.resize(N);
-    std::vector<double> V(N * N);
+    U.assign(N * N, 0.0);
     for (size_t k = 0; k < N; ++k) {
-        w[k] = a(order[k], order[k]);
-        for (size_t i = 0; i < N; ++i) V[i * N + k] = U[i * N + order[k]];
+        w[k] = d[order[k]];
+        const double* col = &QT[order[k] * N];
+        for (size_t i = 0; i < N; ++i) U[i * N + k] = col[i];
     }
-    U.swap(V);
 }
 
 }  // namespace fe

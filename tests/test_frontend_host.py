@@ -1,6 +1,9 @@
 """Host side of the product's integral front end, checked against the oracle
 (no GPU): the GENBAS reader, shells, one- and two-electron integrals of
-libaqamd.so, and the water-cluster workload constructor."""
+libaqamd.so, the water-cluster workload constructor, the index map of the
+device ERI expansion and the SCF's eigensolver."""
+
+import os
 
 import numpy as np
 import pytest
@@ -123,3 +126,21 @@ def test_compact_eri_index_is_in_bounds_and_symmetric():
     classes = {min(((p, q, r, s), (q, p, r, s), (p, q, s, r), (q, p, s, r), (r, s, p, q), (s, r, p, q),
                     (r, s, q, p), (s, r, q, p))) for p, q, r, s in itertools.product(range(n), repeat=4)}
     assert len(used) == len(classes)
+
+
+def test_host_eigensolver(tmp_path):
+    """The SCF's symmetric eigensolver (Householder + implicit QL): residual,
+    orthogonality and ordering at machine precision, degenerate spectra
+    included (C++ check compiled against the built library)."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    here = os.path.dirname(os.path.abspath(__file__))
+    libdir = os.path.join(os.path.dirname(here), "aquarius_amd")
+    exe = str(tmp_path / "eigh_check")
+    subprocess.run([gxx, "-std=c++17", "-O2", os.path.join(here, "cpp", "eigh_check.cpp"), "-o", exe,
+                    f"-L{libdir}", "-laqamd", f"-Wl,-rpath,{libdir}"], check=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout
