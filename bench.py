@@ -34,9 +34,11 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 # HBM bytes per launch of a kernel class, from two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE)
 # of `bench.py --only fp64 --steps 1 --warmup 1 --autotune 0`, scaled by the same counters on a plain 1 GiB copy
 # (fetch x1.9999, write x1.0); mean over the class's launches in the timed iteration.  Keyed by
-# (timing class, o, v); the per-launch table and the commands are in the profile summary
-# gemm_f64_128x128_NT_o48v288_hbm_traffic.txt.
-TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 87.7e9}
+# (timing class, o, v); the per-launch tables and the commands are in the profile summaries
+# gemm_f64_128x128_NT_o48v288_hbm_traffic.txt and gemm_bf16x3_256x256_o48v288_hbm_traffic.txt
+# (the BF16x3 class from `bench.py --only mixed --steps 1 --warmup 1`, the timed iteration).
+TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 87.7e9,
+                      ("gemm_bf16x3_256x256_TN", 48, 288): 20.2e9}
 
 
 def flops_per_iter(o, v):
