@@ -7,9 +7,11 @@ Follows, statement by statement, the reference's
   * DIIS::extrapolate        src/convergence/diis.hpp:16-227
   * Iterative::run           src/util/iterative.hpp:126-188
   * Hbar products            src/cc/rhfccsd.cxx:108-188 (RHFCCSD.hbar; pinned by
-                             the T = 0 limit, FME = dE/dT1 / 2 and the dependence
-                             on T1 of the first-order terms -- the remaining
-                             T-dependent terms of W are "parity unpinned")
+                             the T = 0 limit, FME = dE/dT1 / 2, and through their
+                             consumer: the EOM-EE-CCSD singlet sigma vector built
+                             from them, src/cc/rhfeomeeccsd.cxx:379-424
+                             (eom_singlet_sigma), equals the exact derivative of
+                             the CC residual along any R at converged T)
   * Denominator              src/operator/denominator.hpp:21-89
 with the operator semantics of oracle.ctf_ops (CTFTensor, ctf_tensor.cxx:403-514).
 
@@ -205,6 +207,41 @@ class RHFCCSD:
 
     # rhfccsd.cxx:194-327
     def iterate(self):
+        self._residual()
+        T1, T2, Z1, Z2, Tau = self.T1, self.T2, self.Z1, self.Z2, self.Tau
+        fAI, VABIJSA = self.fAI, self.VABIJSA
+        dA, dI = self.D.dA, self.D.dI
+        ops.weight(Z1, [dA, dI])                                          # :313
+        ops.weight(Z2, [dA, dA, dI, dI])                                  # :314
+        if self.config.get("freeze_t1"):
+            Z1[...] = 0.0
+        T1 += Z1                                                          # :316
+        T2 += Z2                                                          # :317
+
+        tsum(1.0, T2, "abij", 0.0, Tau, "abij")                           # :319
+        mult(1.0, T1, "ai", T1, "bj", 1.0, Tau, "abij")                    # :320
+
+        self.energy = 2 * ops.dot(fAI, "ai", T1, "ai") + ops.dot(VABIJSA, "abij", Tau, "abij")  # :322
+        self.conv = max(ops.norm(Z1, 0), ops.norm(Z2, 0))                 # :323
+        self.diis.extrapolate([T1, T2], [Z1, Z2])                         # :325-326
+
+    def residual_at(self, T1, T2):
+        """The unweighted residual (Z1, Z2) of rhfccsd.cxx:231-309 at the given
+        amplitudes; the solver's own state is left as it was."""
+        names = ("T1", "T2", "Z1", "Z2", "Tau", "T2SA", "TauSA", "FAE", "FMI", "FME", "WMNIJ", "WMNEJ",
+                 "WAMIJ", "WAMEI", "WAMIE")
+        saved = {k: getattr(self, k).copy() for k in names}
+        try:
+            self.T1[...] = T1
+            self.T2[...] = T2
+            self._residual()
+            return self.Z1.copy(), self.Z2.copy()
+        finally:
+            for k, x in saved.items():
+                getattr(self, k)[...] = x
+
+    # rhfccsd.cxx:231-309: intermediates and the residual Z1, Z2 of the current T
+    def _residual(self):
         V = self.V
         fAI, fAB, fIJ = self.fAI, self.fAB, self.fIJ
         VABCD, VABCI, VABIJ = V["VABCD"], V["VABCI"], V["VABIJ"]
@@ -283,20 +320,6 @@ class RHFCCSD:
         mult(-0.5, WAMEI, "amei", T2, "ebjm", 1.0, Z2, "abij")            # :306
         mult(-1.0, WAMEI, "amej", T2, "ebim", 1.0, Z2, "abij")            # :307
         tsum(1.0, Z2, "baji", 1.0, Z2, "abij")                            # :309 (aliased)
-
-        ops.weight(Z1, [dA, dI])                                          # :313
-        ops.weight(Z2, [dA, dA, dI, dI])                                  # :314
-        if self.config.get("freeze_t1"):
-            Z1[...] = 0.0
-        T1 += Z1                                                          # :316
-        T2 += Z2                                                          # :317
-
-        tsum(1.0, T2, "abij", 0.0, Tau, "abij")                           # :319
-        mult(1.0, T1, "ai", T1, "bj", 1.0, Tau, "abij")                    # :320
-
-        self.energy = 2 * ops.dot(fAI, "ai", T1, "ai") + ops.dot(VABIJSA, "abij", Tau, "abij")  # :322
-        self.conv = max(ops.norm(Z1, 0), ops.norm(Z2, 0))                 # :323
-        self.diis.extrapolate([T1, T2], [Z1, Z2])                         # :325-326
 
     # rhfccsd.cxx:108-188
     def hbar(self, with_wabej=False):
@@ -396,3 +419,39 @@ class RHFCCSD:
                 callback(self)
             self.iter += 1
         return self.converged()
+
+
+def eom_singlet_sigma(cc, R1, R2):
+    """Singlet sigma vector of the EOM-EE-CCSD consumer of the Hbar products,
+    restated from src/cc/rhfeomeeccsd.cxx:381-424 (the `else` branch, the
+    singlet of RHFEOMEECCSD<U>::iterate), with the products of cc.hbar(True):
+    ccsd.F(AE/MI/ME) = FAE/FMI/FME, ccsd.W(Am,Ie) = WAMIE (WAMIESA there),
+    ccsd.W(Am,Ei) = WAMEI, ccsd.W(Mn,Ej) = WMNEJ, ccsd.W(Ab,Ej) = WABEJ,
+    ccsd.W(Am,Ij) = WAMIJ, ccsd.W(Mn,Ij) = WMNIJ.  Returns (Z1, Z2)."""
+    V = cc.V
+    VABCD, VABCI, VABIJ = V["VABCD"], V["VABCI"], V["VABIJ"]
+    T1, T2 = cc.T1, cc.T2
+    Tau = T2 + np.einsum("ai,bj->abij", T1, T1)                                  # :117-118
+    FAE, FMI, FME = cc.FAE, cc.FMI, cc.FME
+    WAMIESA, WAMEI, WMNEJ, WABEJ, WAMIJ, WMNIJ = cc.WAMIE, cc.WAMEI, cc.WMNEJ, cc.WABEJ, cc.WAMIJ, cc.WMNIJ
+    es = np.einsum
+    R2SA = 2 * R2 - R2.transpose(0, 1, 3, 2)                                     # :381-382
+    XMI = es("efmn,efin->mi", VABIJ, R2SA)                                       # :384
+    Z1 = (es("ae,ei->ai", FAE, R1) - es("mi,am->ai", FMI, R1) - es("mi,am->ai", XMI, T1)
+          + es("amie,em->ai", WAMIESA, R1) + es("me,aeim->ai", FME, R2SA)
+          + es("efam,efim->ai", VABCI, R2SA) - es("mnei,eamn->ai", WMNEJ, R2SA))   # :386-392
+    XMI = XMI + 2 * es("nmei,en->mi", WMNEJ, R1) - es("mnei,en->mi", WMNEJ, R1)  # :394-395
+    XME = 2 * es("efmn,fn->me", VABIJ, R1) - es("efnm,fn->me", VABIJ, R1)        # :397-398
+    XAE = (2 * es("efam,fm->ae", VABCI, R1) - es("feam,fm->ae", VABCI, R1)
+           - es("me,am->ae", XME, T1) - es("efmn,afmn->ae", VABIJ, R2SA))        # :400-403
+    XAMIJ = es("efam,efij->amij", VABCI, R2)                                     # :405
+    XMNIJ = es("efmn,efij->mnij", VABIJ, R2)                                     # :407
+    Z2 = (es("abej,ei->abij", WABEJ, R1) - es("amij,bm->abij", WAMIJ, R1)
+          - es("amij,bm->abij", XAMIJ, T1) + es("ae,ebij->abij", FAE, R2)
+          - es("mi,abmj->abij", FMI, R2) + es("ae,ebij->abij", XAE, T2)
+          - es("mi,abmj->abij", XMI, T2) + 0.5 * es("mnij,abmn->abij", WMNIJ, R2)
+          + 0.5 * es("mnij,abmn->abij", XMNIJ, Tau) + 0.5 * es("abef,efij->abij", VABCD, R2)
+          + 0.5 * es("amie,ebmj->abij", WAMIESA, R2SA) - 0.5 * es("amei,ebjm->abij", WAMEI, R2)
+          - es("amej,ebim->abij", WAMEI, R2))                                    # :409-421
+    Z2 = Z2 + Z2.transpose(1, 0, 3, 2)                                           # :423
+    return Z1, Z2

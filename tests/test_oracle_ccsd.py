@@ -303,3 +303,44 @@ def test_hbar_shapes_and_wabej_order():
     for k in HBAR[:-1]:
         np.testing.assert_array_equal(getattr(a, k), getattr(b, k))
     assert b.WABEJ.shape == (5, 5, 5, 3)
+
+
+def test_hbar_products_pinned_by_their_eom_consumer():
+    """At converged amplitudes the EOM matrix minus E_cc is the Jacobian of the
+    CC residual: for any R, the singlet sigma vector that the reference's
+    EOM-EE-CCSD builds from the Hbar products (rhfeomeeccsd.cxx:381-424) must
+    equal the derivative of the residual Z(T) (rhfccsd.cxx:231-309) along R.
+    Both are polynomials of degree <= 4 in the amplitudes, so the five-point
+    central difference is exact up to rounding.  This pins every product the
+    consumer reads: FAE, FMI, FME, WMNIJ, WMNEJ, WAMIJ, WAMEI, WAMIE, WABEJ."""
+    f, V, _, _ = synth.synthetic_system(17, 3, 9)
+    cfg = rhfccsd.default_config()
+    cfg["convergence"] = 1e-13
+    cfg["max_iterations"] = 200
+    cfg["diis"]["order"] = 8
+    cc = rhfccsd.RHFCCSD(f, V, cfg)
+    assert cc.run()
+    cc.hbar(with_wabej=True)
+    rng = np.random.default_rng(4)
+    v, o = cc.T1.shape
+    for trial in range(3):
+        R1 = rng.standard_normal((v, o))
+        R2 = rng.standard_normal((v, v, o, o))
+        R2 = R2 + R2.transpose(1, 0, 3, 2)                  # the symmetry of T2
+        s1, s2 = rhfccsd.eom_singlet_sigma(cc, R1, R2)
+        h = 1e-3
+        Z = {k: cc.residual_at(cc.T1 + k * h * R1, cc.T2 + k * h * R2) for k in (-2, -1, 1, 2)}
+        d1 = (-Z[2][0] + 8 * Z[1][0] - 8 * Z[-1][0] + Z[-2][0]) / (12 * h)
+        d2 = (-Z[2][1] + 8 * Z[1][1] - 8 * Z[-1][1] + Z[-2][1]) / (12 * h)
+        scale = max(np.abs(s1).max(), np.abs(s2).max())
+        # measured: ~2e-15 relative (rounding of the difference quotient)
+        assert np.abs(s1 - d1).max() <= 1e-12 * scale, (trial, np.abs(s1 - d1).max(), scale)
+        assert np.abs(s2 - d2).max() <= 1e-12 * scale, (trial, np.abs(s2 - d2).max(), scale)
+    # every product matters: a relative error of 1e-6 in any one of them is detected
+    for name in ("FAE", "FMI", "FME", "WMNIJ", "WMNEJ", "WAMIJ", "WAMEI", "WAMIE", "WABEJ"):
+        x = getattr(cc, name)
+        keep = x.copy()
+        x *= 1.0 + 1e-6
+        t1, t2 = rhfccsd.eom_singlet_sigma(cc, R1, R2)
+        x[...] = keep
+        assert max(np.abs(t1 - d1).max(), np.abs(t2 - d2).max()) > 1e-12 * scale, name
