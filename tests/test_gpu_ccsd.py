@@ -201,3 +201,37 @@ def test_hbar_products_match_oracle(ctx):
         assert np.abs(got - want).max() <= 1e-12 * scale, k
     cc.close()
     ints.free()
+
+
+def test_device_hbar_satisfies_the_eom_jacobian_identity(ctx):
+    """The device's converged amplitudes and Hbar products, fed to the EOM-EE
+    singlet sigma vector (rhfeomeeccsd.cxx:379-424), reproduce the exact
+    derivative of the CC residual (rhfccsd.cxx:231-309) along random R."""
+    print("case: device Hbar through the EOM Jacobian identity, o=3 v=11", flush=True)
+    o, v = 3, 11
+    f0, V, _, _ = synth.synthetic_system(12, o, v)
+    f = noncanonical(f0, V, 12, o, v)
+    ints = aq.Integrals.from_numpy(ctx, f, V)
+    cc = aq.RHFCCSD(ctx, ints, convergence=1e-12, diis_order=8, max_iterations=200)
+    assert cc.run()["converged"]
+    cc.hbar(with_wabej=True)
+    ref = rhfccsd.RHFCCSD(f, V, oracle_cfg(1e-12, 8))
+    ref.T1[...] = cc.tensor("T1").numpy()
+    ref.T2[...] = cc.tensor("T2").numpy()
+    for k in ("FAE", "FMI", "FME", "WMNIJ", "WMNEJ", "WAMIJ", "WAMEI", "WAMIE"):
+        getattr(ref, k)[...] = cc.tensor(k).numpy()
+    ref.WABEJ = cc.tensor("WABEJ").numpy()
+    rng = np.random.default_rng(6)
+    R1 = rng.standard_normal((v, o))
+    R2 = rng.standard_normal((v, v, o, o))
+    R2 = R2 + R2.transpose(1, 0, 3, 2)
+    s1, s2 = rhfccsd.eom_singlet_sigma(ref, R1, R2)
+    h = 1e-3
+    Z = {k: ref.residual_at(ref.T1 + k * h * R1, ref.T2 + k * h * R2) for k in (-2, -1, 1, 2)}
+    d1 = (-Z[2][0] + 8 * Z[1][0] - 8 * Z[-1][0] + Z[-2][0]) / (12 * h)
+    d2 = (-Z[2][1] + 8 * Z[1][1] - 8 * Z[-1][1] + Z[-2][1]) / (12 * h)
+    scale = max(np.abs(s1).max(), np.abs(s2).max())
+    # the device's T satisfies the residual to ~1e-12, which bounds the identity's defect
+    assert np.abs(s1 - d1).max() <= 1e-9 * scale and np.abs(s2 - d2).max() <= 1e-9 * scale
+    cc.close()
+    ints.free()
