@@ -477,6 +477,8 @@ _SIGS.update({
     "aq_server_drain": [_P],
     "aq_server_stats_get": [_P, C.POINTER(ServerStats)],
     "aq_server_destroy": [_P],
+    "aq_server_set_grouping": [_P, C.c_int],
+    "aq_server_grouping_stats": [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)],
 })
 _RESTYPE["aq_job_bytes_estimate"] = C.c_size_t
 
@@ -516,6 +518,16 @@ class Server:
         s = ServerStats()
         _ck(lib().aq_server_stats_get(self.h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def set_grouping(self, window_us):
+        """Cross-job grouped GEMM launches with a host wait of at most
+        window_us per GEMM (0: off); only while no job is queued or running."""
+        _ck(lib().aq_server_set_grouping(self.h, int(window_us)))
+
+    def grouping_stats(self):
+        n, g, mx = C.c_int64(), C.c_int64(), C.c_int()
+        _ck(lib().aq_server_grouping_stats(self.h, C.byref(n), C.byref(g), C.byref(mx)))
+        return {"launches": n.value, "gemms": g.value, "max_group": mx.value}
 
     def close(self):
         if self.h:

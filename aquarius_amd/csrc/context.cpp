@@ -238,6 +238,7 @@ int aq_ctx_destroy(aq_ctx* ctx) {
         pool_release_all(ctx);
         for (auto& kv : ctx->cache) dfree(ctx, kv.second.ptr);
         for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
+        if (ctx->group_ready) (void)hipEventDestroy(ctx->group_ready);
         dfree(ctx, ctx->d_partials);
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipHostFree(ctx->h_result);

@@ -27,7 +27,9 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <map>
 #include <memory>
@@ -227,8 +229,11 @@ struct FastLoader {
     }
 };
 
-template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int BK, int MINW>
-__global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
+// One BM x BN output tile of batch/split slice bz: block bx of a launch of
+// this GEMM alone (REMAP: XCD-aware order), or tile bx (tile_of order) of a
+// grouped launch.
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int BK, bool REMAP>
+__device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) {
     constexpr int PAD = 1;
     constexpr int WTM = BM / WM, WTN = BN / WN;   // wave tile
     constexpr int FM = WTM / 16, FN = WTN / 16;   // 16x16 blocks per wave
@@ -244,11 +249,10 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
 
     const int64_t tilesN = (g.N + BN - 1) / BN;
     const int64_t tilesM = (g.M + BM - 1) / BM;
-    const int64_t t = remap_tile(blockIdx.x, tilesM * tilesN);
+    const int64_t t = REMAP ? remap_tile(bx, tilesM * tilesN) : bx;
     int64_t tmi, tni;
     tile_of(t, tilesM, tilesN, g.raster, tmi, tni);
     const int64_t m0 = tmi * BM, n0 = tni * BN;
-    const int bz = blockIdx.z;
     const int b = bz / g.nsplit, split = bz % g.nsplit;
     const int64_t kbeg = int64_t(split) * g.kchunk;
     const int64_t kend = min(g.K, kbeg + g.kchunk);
@@ -360,13 +364,18 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
     }
 }
 
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int BK, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
+    gemm_tile<T, BM, BN, WM, WN, TA, TB, BK, true>(g, blockIdx.x, blockIdx.z);
+}
+
 // ------------------------------------------------------------------ BF16x3
 
 // PF = 1: the next k-tile's global loads are issued before this tile's
 // MFMAs.  PF = 2: two register sets, the tile two steps ahead is loaded, so a
 // load has two tiles of MFMA work (~1500 cycles) to arrive instead of one.
-template <int BM, int BN, int WM, int WN, int PF>
-__global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g) {
+template <int BM, int BN, int WM, int WN, int PF, bool REMAP>
+__device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, int bz) {
     constexpr int BK = 32;
     // LDS rows of BK bf16 (64 bytes, four 16-byte chunks), chunk c of row r
     // stored at chunk c ^ ((r >> 1) & 3).  Measured on gfx950 (bank-conflict
@@ -394,11 +403,10 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
 
     const int64_t tilesN = (g.N + BN - 1) / BN;
     const int64_t tilesM = (g.M + BM - 1) / BM;
-    const int64_t t = remap_tile(blockIdx.x, tilesM * tilesN);
+    const int64_t t = REMAP ? remap_tile(bx, tilesM * tilesN) : bx;
     int64_t tmi, tni;
     tile_of(t, tilesM, tilesN, g.raster, tmi, tni);
     const int64_t m0 = tmi * BM, n0 = tni * BN;
-    const int bz = blockIdx.z;
     const int b = bz / g.nsplit, split = bz % g.nsplit;
     const int64_t kbeg = int64_t(split) * g.kchunk;
     const int64_t kend = min(g.K, kbeg + g.kchunk);
@@ -546,6 +554,50 @@ __global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g
             }
         }
     }
+}
+
+template <int BM, int BN, int WM, int WN, int PF>
+__global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g) {
+    gemm_tile_bf16x3<BM, BN, WM, WN, PF, true>(g, blockIdx.x, blockIdx.z);
+}
+
+// ---------------------------------------------------------- grouped launch
+//
+// One launch over the output tiles of several independent GEMMs (problems
+// from different contexts with the same format, operand layouts and block
+// shape).  Problem p owns blocks [first[p], first[p+1]) of the 1-D grid,
+// ordered as its own (tile, batch x split) grid; every block runs the same
+// tile body as the single-problem kernels, so each element is computed with
+// the same operations in the same order as in a launch of its GEMM alone.
+constexpr int kMaxGroup = 8;
+struct GemmGroup {
+    GemmArgs p[kMaxGroup];
+    int64_t first[kMaxGroup + 1];
+    int n;
+};
+
+__device__ __forceinline__ int group_problem(const GemmGroup& G, int64_t bid) {
+    int p = 0;
+    while (p + 1 < G.n && bid >= G.first[p + 1]) ++p;
+    return p;
+}
+
+template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int BK, int MINW>
+__global__ __launch_bounds__(256, MINW) void k_gemm_grouped(GemmGroup G) {
+    const int64_t bid = blockIdx.x;
+    const int p = group_problem(G, bid);
+    const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
+    const int64_t local = bid - G.first[p];
+    gemm_tile<T, BM, BN, WM, WN, TA, TB, BK, false>(G.p[p], local % tiles, int(local / tiles));
+}
+
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_grouped(GemmGroup G) {
+    const int64_t bid = blockIdx.x;
+    const int p = group_problem(G, bid);
+    const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
+    const int64_t local = bid - G.first[p];
+    gemm_tile_bf16x3<BM, BN, WM, WN, 1, false>(G.p[p], local % tiles, int(local / tiles));
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -709,6 +761,193 @@ static int tuned_variant(aq_ctx* ctx, const GemmDesc& d, int BM, int BN, const G
     return e.variant >= 0 ? e.variant : defv;
 }
 
+// ------------------------------------------------------- grouping (host)
+
+namespace {
+
+struct GroupKey {
+    int fmt, ta, tb, bm, bn;
+    bool operator==(const GroupKey& o) const {
+        return fmt == o.fmt && ta == o.ta && tb == o.tb && bm == o.bm && bn == o.bn;
+    }
+};
+
+// block shapes with a grouped kernel: the 128 x 128 and 64 x 64 FP64 and
+// BF16x3 tiles (the FP32 comparison path and the skinny tiles launch alone)
+bool group_shape(Fmt fmt, int BM, int BN) {
+    return (fmt == FMT_F64 || fmt == FMT_BF16X2) && BM == BN && (BM == 128 || BM == 64);
+}
+
+void launch_grouped(const GroupKey& k, const GemmGroup& G, int64_t blocks, hipStream_t s) {
+    const dim3 grid{unsigned(blocks)}, block{256};
+#define AQ_GROUPED(BM_, BK_, MW_)                                                                                      \
+    do {                                                                                                           \
+        if (k.ta && k.tb)                                                                                          \
+            hipLaunchKernelGGL((k_gemm_grouped<double, BM_, BM_, 2, 2, true, true, BK_, MW_>), grid, block, 0, s, G);   \
+        else if (k.ta)                                                                                             \
+            hipLaunchKernelGGL((k_gemm_grouped<double, BM_, BM_, 2, 2, true, false, BK_, MW_>), grid, block, 0, s, G);  \
+        else if (k.tb)                                                                                             \
+            hipLaunchKernelGGL((k_gemm_grouped<double, BM_, BM_, 2, 2, false, true, BK_, MW_>), grid, block, 0, s, G);  \
+        else                                                                                                       \
+            hipLaunchKernelGGL((k_gemm_grouped<double, BM_, BM_, 2, 2, false, false, BK_, MW_>), grid, block, 0, s, G); \
+    } while (0)
+    if (k.fmt == FMT_F64) {
+        if (k.bm == 128) AQ_GROUPED(128, 16, 2);   // the default FP64 variant's kernel
+        else AQ_GROUPED(64, 16, 1);
+    } else if (k.bm == 128) {
+        hipLaunchKernelGGL((k_gemm_bf16x3_grouped<128, 128, 2, 2>), grid, block, 0, s, G);
+    } else {
+        hipLaunchKernelGGL((k_gemm_bf16x3_grouped<64, 64, 2, 2>), grid, block, 0, s, G);
+    }
+#undef AQ_GROUPED
+}
+
+}  // namespace
+
+struct GemmBatcher {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::chrono::microseconds window{0};
+    std::mutex mu;
+    std::condition_variable cv;
+    int in_stage = 0;                     // contexts inside a contraction stage
+    struct Launch {
+        hipEvent_t done = nullptr;        // recorded after the grouped kernel
+        int refs = 0;                     // members that have not yet waited on it
+    };
+    struct Item {
+        GroupKey key;
+        GemmArgs a;
+        int64_t blocks;
+        hipEvent_t ready;                 // the member's operands are written
+        Launch* launch = nullptr;
+        bool failed = false;
+    };
+    std::vector<Item*> pending;           // in submission order
+    std::vector<hipEvent_t> spare;
+    int64_t launches = 0, problems = 0;
+    int max_group = 0;
+};
+
+// Launch up to kMaxGroup pending items of `key` as one grouped kernel (mu held).
+static void launch_group(GemmBatcher* b, const GroupKey& key) {
+    std::vector<GemmBatcher::Item*> take;
+    for (auto it = b->pending.begin(); it != b->pending.end() && int(take.size()) < kMaxGroup;) {
+        if ((*it)->key == key) {
+            take.push_back(*it);
+            it = b->pending.erase(it);
+        } else {
+            ++it;
+        }
+    }
+    if (take.empty()) return;
+    auto* L = new GemmBatcher::Launch();
+    try {
+        GemmGroup G{};
+        G.n = int(take.size());
+        int64_t blocks = 0;
+        for (int i = 0; i < G.n; ++i) {
+            G.p[i] = take[size_t(i)]->a;
+            G.first[i] = blocks;
+            blocks += take[size_t(i)]->blocks;
+            AQ_HIP(hipStreamWaitEvent(b->stream, take[size_t(i)]->ready, 0));
+        }
+        G.first[G.n] = blocks;
+        AQ_CHECK(blocks > 0 && blocks < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "grouped GEMM grid too large");
+        if (!b->spare.empty()) {
+            L->done = b->spare.back();
+            b->spare.pop_back();
+        } else {
+            AQ_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
+        }
+        launch_grouped(key, G, blocks, b->stream);
+        AQ_HIP(hipGetLastError());
+        AQ_HIP(hipEventRecord(L->done, b->stream));
+    } catch (...) {
+        if (L->done) b->spare.push_back(L->done);
+        delete L;
+        for (auto* it : take) it->failed = true;
+        b->cv.notify_all();
+        throw;
+    }
+    L->refs = int(take.size());
+    for (auto* it : take) it->launch = L;
+    b->launches += 1;
+    b->problems += int64_t(take.size());
+    b->max_group = std::max(b->max_group, int(take.size()));
+    b->cv.notify_all();
+}
+
+// Hand one GEMM (arguments final, split-K partials allocated) to the batcher:
+// returns once it is launched, with ctx->stream ordered after the grouped kernel.
+static void batcher_submit(aq_ctx* ctx, const GroupKey& key, const GemmArgs& a, int64_t blocks) {
+    GemmBatcher* b = ctx->batcher;
+    if (!ctx->group_ready) AQ_HIP(hipEventCreateWithFlags(&ctx->group_ready, hipEventDisableTiming));
+    AQ_HIP(hipEventRecord(ctx->group_ready, ctx->stream));
+    GemmBatcher::Item item{key, a, blocks, ctx->group_ready};
+    std::unique_lock<std::mutex> lk(b->mu);
+    b->pending.push_back(&item);
+    struct Unlist {   // an item still pending when this frame unwinds is withdrawn (mu held)
+        GemmBatcher* b;
+        GemmBatcher::Item* it;
+        ~Unlist() {
+            auto pos = std::find(b->pending.begin(), b->pending.end(), it);
+            if (pos != b->pending.end()) b->pending.erase(pos);
+        }
+    } unlist{b, &item};
+    const auto deadline = std::chrono::steady_clock::now() + b->window;
+    while (!item.launch && !item.failed) {
+        int same = 0;
+        for (const auto* x : b->pending) same += x->key == key ? 1 : 0;
+        const int want = std::min(std::max(b->in_stage, 1), kMaxGroup);
+        if (same >= want || std::chrono::steady_clock::now() >= deadline) {
+            launch_group(b, key);
+            continue;
+        }
+        b->cv.wait_until(lk, deadline);
+    }
+    AQ_CHECK(!item.failed, AQ_ERR_HIP, "grouped GEMM launch failed");
+    GemmBatcher::Launch* L = item.launch;
+    const hipError_t e = hipStreamWaitEvent(ctx->stream, L->done, 0);
+    if (--L->refs == 0) {
+        b->spare.push_back(L->done);
+        delete L;
+    }
+    AQ_HIP(e);
+}
+
+GemmBatcher* batcher_create(int device, int window_us) {
+    AQ_CHECK(window_us >= 0, AQ_ERR_INVALID, "grouping window must be >= 0 us");
+    std::unique_ptr<GemmBatcher> b(new GemmBatcher());
+    b->device = device;
+    b->window = std::chrono::microseconds(window_us);
+    AQ_HIP(hipSetDevice(device));
+    AQ_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    return b.release();
+}
+
+void batcher_destroy(GemmBatcher* b) {
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    (void)hipStreamSynchronize(b->stream);
+    for (auto e : b->spare) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+void batcher_stage(GemmBatcher* b, int delta) {
+    std::lock_guard<std::mutex> lk(b->mu);
+    b->in_stage += delta;
+    b->cv.notify_all();
+}
+
+void batcher_stats(GemmBatcher* b, int64_t* launches, int64_t* problems, int* max_group) {
+    std::lock_guard<std::mutex> lk(b->mu);
+    if (launches) *launches = b->launches;
+    if (problems) *problems = b->problems;
+    if (max_group) *max_group = b->max_group;
+}
+
 void gemm(aq_ctx* ctx, const GemmDesc& d) {
     AQ_CHECK(d.M > 0 && d.N > 0 && d.K > 0 && d.batch > 0, AQ_ERR_INVALID, "empty GEMM");
     AQ_CHECK(d.batch < 65536, AQ_ERR_UNSUPPORTED, "GEMM batch too large");
@@ -769,6 +1008,19 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     // groups of 8 tile rows: measured on the 13824^3 ring GEMMs, BF16x3 279 -> 357 TFLOP/s
     // (FP64 unchanged in time, MFMA-bound, with less L2-to-fabric traffic)
     a.raster = 8;
+
+    if (ctx->batcher && !ctx->capture && group_shape(d.fmt, BM, BN)) {
+        std::unique_ptr<Scratch> part;
+        if (nsplit > 1) {
+            part.reset(new Scratch(ctx, size_t(nsplit) * d.batch * d.M * d.N * 8));
+            a.P = part->as<double>();
+        }
+        batcher_submit(ctx, GroupKey{int(d.fmt), d.transA ? 1 : 0, d.transB ? 1 : 0, BM, BN}, a,
+                       base_tiles * d.batch * nsplit);
+        if (nsplit > 1)
+            launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
+        return;
+    }
 
     int variant = ctx->gemm_variant[d.fmt];
     if (variant < 0) variant = ctx->autotune ? tuned_variant(ctx, d, BM, BN, a) : default_gemm_variant(d.fmt);

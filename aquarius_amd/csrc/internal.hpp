@@ -108,6 +108,10 @@ struct aq_tensor {
     uint64_t version = 0;   // bumped by every write; keys the operand cache
 };
 
+namespace aqamd {
+struct GemmBatcher;
+}
+
 struct aq_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -139,6 +143,13 @@ struct aq_ctx {
     // then nothing may allocate
     uint64_t mem_gen = 0;
     aqamd::CaptureLog* capture = nullptr;
+
+    // cross-context GEMM grouping (a server's slot contexts): while `batcher`
+    // is set, eager GEMMs of the grouped block shapes join other contexts'
+    // GEMMs in one launch on the batcher's stream; group_ready marks this
+    // stream's position at a submission
+    aqamd::GemmBatcher* batcher = nullptr;
+    hipEvent_t group_ready = nullptr;
 
     // timing
     bool timing = false;
@@ -262,5 +273,22 @@ struct GemmDesc {
 };
 int autotune_entries(aq_tune_entry* out, int cap);   // gemm.hip
 void gemm(aq_ctx* ctx, const GemmDesc& d);
+
+// ---- cross-context GEMM grouping (gemm.hip).  A submission waits on the host
+// until every context inside a contraction stage has a GEMM of the same key
+// pending, or `window_us` has passed, then one grouped kernel runs them all.
+GemmBatcher* batcher_create(int device, int window_us);
+void batcher_destroy(GemmBatcher* b);
+void batcher_stage(GemmBatcher* b, int delta);   // a context enters (+1) or leaves (-1) a stage
+void batcher_stats(GemmBatcher* b, int64_t* launches, int64_t* problems, int* max_group);
+
+// Marks the calling context as inside a contraction stage for its batcher.
+struct StageMark {
+    GemmBatcher* b;
+    explicit StageMark(GemmBatcher* b_) : b(b_) { if (b) batcher_stage(b, +1); }
+    ~StageMark() { if (b) batcher_stage(b, -1); }
+    StageMark(const StageMark&) = delete;
+    StageMark& operator=(const StageMark&) = delete;
+};
 
 }  // namespace aqamd

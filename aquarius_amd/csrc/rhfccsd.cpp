@@ -308,7 +308,10 @@ struct aq_rhfccsd {
             return;
         }
         if (want && graph.warm_prec == prec && capture_stage(prec)) return;
-        contractions();
+        {
+            StageMark mark(ctx->batcher);
+            contractions();
+        }
         graph.warm_prec = prec;
     }
 

@@ -8,7 +8,9 @@
 // except that once the oldest queued job has waited longer than
 // max_bypass_s no later job may overtake it.  Jobs run to convergence with
 // their integrals, amplitudes and DIIS history resident on the device; their
-// kernels overlap on the GPU through the per-slot streams.  The slot contexts
+// kernels overlap on the GPU through the per-slot streams (and, with grouping
+// on, same-key GEMMs of different jobs run as one grouped launch on the
+// batcher's stream; see aq_server_set_grouping).  The slot contexts
 // autotune their GEMMs: the first job of each (o, v) size meets new GEMM shape
 // buckets and times the kernel variants once; later jobs reuse the choices.  Nothing mutable is
 // shared between jobs, so each job's arithmetic is independent of the mix.
@@ -42,6 +44,7 @@ struct aq_server {
     size_t budget = 0, reserved = 0;
     double max_bypass_s = 0.0;
     std::vector<aq_ctx*> ctxs;
+    GemmBatcher* batcher = nullptr;   // cross-job GEMM grouping (null: off)
     std::vector<std::thread> workers;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -282,6 +285,34 @@ int aq_server_stats_get(aq_server* s, aq_server_stats* out) {
     });
 }
 
+int aq_server_set_grouping(aq_server* s, int window_us) {
+    return guarded([&] {
+        AQ_CHECK(s, AQ_ERR_INVALID, "null server");
+        AQ_CHECK(window_us >= 0, AQ_ERR_INVALID, "grouping window must be >= 0 us");
+        std::lock_guard<std::mutex> lk(s->mu);
+        AQ_CHECK(s->queue.empty() && s->running == 0, AQ_ERR_INVALID,
+                 "grouping can change only while no job is queued or running");
+        GemmBatcher* nb = window_us > 0 ? batcher_create(s->device, window_us) : nullptr;
+        for (auto* c : s->ctxs) c->batcher = nb;
+        batcher_destroy(s->batcher);
+        s->batcher = nb;
+    });
+}
+
+int aq_server_grouping_stats(aq_server* s, int64_t* launches, int64_t* gemms, int* max_group) {
+    return guarded([&] {
+        AQ_CHECK(s, AQ_ERR_INVALID, "null server");
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->batcher) {
+            batcher_stats(s->batcher, launches, gemms, max_group);
+        } else {
+            if (launches) *launches = 0;
+            if (gemms) *gemms = 0;
+            if (max_group) *max_group = 0;
+        }
+    });
+}
+
 int aq_server_destroy(aq_server* s) {
     return guarded([&] {
         if (!s) return;
@@ -293,6 +324,7 @@ int aq_server_destroy(aq_server* s) {
         s->cv_work.notify_all();
         for (auto& t : s->workers) t.join();
         for (auto* c : s->ctxs) aq_ctx_destroy(c);
+        batcher_destroy(s->batcher);
         delete s;
     });
 }

@@ -127,6 +127,8 @@ def serving(a):
                             refine_below=a.refine_below, min_reduced_flops=1e8)
     res = []
     with aq.Server(slots=a.slots, hbm_budget=int(a.budget_gb * 2**30), max_bypass_s=a.max_bypass) as srv:
+        if a.grouping_us > 0:
+            srv.set_grouping(a.grouping_us)
         t0 = time.perf_counter()
         ids = []
         for seed, o, v, t in jobs:
@@ -141,13 +143,16 @@ def serving(a):
                 log(f"served {k + 1}/{len(ids)} (last: o={jobs[k][1]} v={jobs[k][2]} E={r['energy']:.8f} "
                     f"time-to-E {r['t_end'] - r['t_submit']:.2f}s)")
         wall = max(r["t_end"] for r in res) - min(r["t_submit"] for r in res)
+        grouping = srv.grouping_stats()
     ok = [r for r in res if r["status"] == aq.JOB_DONE and r["converged"]]
     tte = np.array([r["t_end"] - r["t_submit"] for r in ok])
     out = {"metric": "served jobs/h + p99 time-to-E", "value": 3600.0 * len(ok) / wall, "unit": "jobs/h",
            "n_gpus": 1, "higher_is_better": True, "dtype": f"{a.mixed}+fp64", "data": "synthetic",
            "config": {"workload": f"{a.jobs} synthetic CCSD jobs, o~U{{{a.o_min}..{a.o_max}}}, v=round(o*U[4,8]), "
                                   f"Poisson arrivals {a.rate}/s, HBM budget {a.budget_gb} GB, {a.slots} slots",
-                      "seed": a.seed, "convergence": a.convergence, "diis_order": a.diis},
+                      "seed": a.seed, "convergence": a.convergence, "diis_order": a.diis,
+                      "graph": a.graph, "grouping_us": a.grouping_us},
+           "grouped_launches": grouping,
            "jobs": len(res), "converged": len(ok),
            "failed": sum(r["status"] == aq.JOB_FAILED for r in res),
            "first_error": next((r["error"] for r in res if r["status"] == aq.JOB_FAILED), ""),
@@ -372,6 +377,9 @@ def main():
     ap.add_argument("--packed-ladder", type=int, default=1,
                     help="particle-particle ladder on packed (anti)symmetric pairs (1) or as the full contraction (0)")
     ap.add_argument("--graph", type=int, default=1, help="serving: HIP-graph replay of each job's contraction stage")
+    ap.add_argument("--grouping-us", type=int, default=0,
+                    help="serving: cross-job grouped GEMM launches with this host window in us (0: off; "
+                         "stages replayed from a graph do not group, so pair it with --graph 0)")
     ap.add_argument("--compare-precisions", type=int, default=1,
                     help="also solve with the other reduced arithmetic (FP32 MFMA beside BF16x3) and report its "
                          "energy and T2 error against FP64")

@@ -52,6 +52,31 @@ def test_alone_equals_in_the_mix(prec):
     assert overlap
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
+def test_grouped_launches_equal_alone(prec):
+    """Cross-job grouped GEMM launches (aq_server_set_grouping): same-key
+    GEMMs of the four slots run as one kernel, and every job's energy and
+    iteration count are bit-identical to the job run alone."""
+    alone = run_alone(JOBS, prec)
+    with aq.Server(slots=4) as s:
+        s.set_grouping(2000)
+        ids = [s.submit(seed, o, v, config=cfg(prec)) for seed, o, v in JOBS]
+        mix = [s.wait(i) for i in ids]
+        st = s.grouping_stats()
+        extra = s.submit(1, 3, 12, config=cfg(prec))
+        with pytest.raises(aq.AqError):
+            s.set_grouping(0)          # refused while a job is queued or running
+        assert s.wait(extra)["status"] == aq.JOB_DONE
+        s.set_grouping(0)
+        assert s.grouping_stats() == {"launches": 0, "gemms": 0, "max_group": 0}
+    print(f"grouping {prec}: {st}")
+    assert st["launches"] > 0 and st["gemms"] > st["launches"] and st["max_group"] >= 2
+    for a, m, (seed, o, v) in zip(alone, mix, JOBS):
+        assert a["status"] == aq.JOB_DONE and m["status"] == aq.JOB_DONE, (a["error"], m["error"])
+        assert a["converged"] and m["converged"]
+        assert a["energy"] == m["energy"] and a["iterations"] == m["iterations"], (seed, a["energy"], m["energy"])
+
+
 def test_served_job_equals_direct_solver():
     seed, o, v = 21, 5, 22
     with aq.Server(slots=2) as s:
