@@ -376,10 +376,10 @@ def main():
     ap.add_argument("--autotune", type=int, default=1, help="GEMM autotuner on (1) or off (0)")
     ap.add_argument("--packed-ladder", type=int, default=1,
                     help="particle-particle ladder on packed (anti)symmetric pairs (1) or as the full contraction (0)")
-    ap.add_argument("--graph", type=int, default=1, help="serving: HIP-graph replay of each job's contraction stage")
-    ap.add_argument("--grouping-us", type=int, default=0,
-                    help="serving: cross-job grouped GEMM launches with this host window in us (0: off; "
-                         "stages replayed from a graph do not group, so pair it with --graph 0)")
+    ap.add_argument("--graph", type=int, default=0, help="serving: HIP-graph replay of each job's contraction stage "
+                    "(stages replayed from a graph do not group)")
+    ap.add_argument("--grouping-us", type=int, default=50,
+                    help="serving: cross-job grouped GEMM launches with this host window in us (0: off)")
     ap.add_argument("--compare-precisions", type=int, default=1,
                     help="also solve with the other reduced arithmetic (FP32 MFMA beside BF16x3) and report its "
                          "energy and T2 error against FP64")
