@@ -467,25 +467,75 @@ __global__ __launch_bounds__(TPB) void k_update2(UpdateArgs u, double* pe, doubl
     const int64_t v = u.v, o = u.o;
     const int64_t n = v * v * o * o;
     double e = 0.0, c = 0.0;
-    for (int64_t idx = int64_t(blockIdx.x) * TPB + threadIdx.x; idx < n; idx += int64_t(gridDim.x) * TPB) {
-        int64_t r = idx;
-        const int a = int(r % v); r /= v;
-        const int b = int(r % v); r /= v;
-        const int i = int(r % o); r /= o;
-        const int j = int(r);
-        double den = 0.0;
-        den += u.dA[a];
-        den += u.dA[b];
-        den += u.dI[i];
-        den += u.dI[j];
-        const double z = fabs(den) < 1e-4 ? 0.0 : u.Z2[idx] / den;
-        u.Z2[idx] = z;
-        const double t = u.T2[idx] + z;
-        u.T2[idx] = t;
-        const double tau = t + u.T1[a + i * v] * u.T1[b + j * v];
-        u.Tau[idx] = tau;
-        e += u.VABIJSA[idx] * tau;
-        c = fmax(c, fabs(z));
+    // (a, b, i, j) digits of the flat index in the mixed radix (v, v, o, o):
+    // split once for the first element and for the grid stride, then advanced
+    // by digit-wise addition with carries (no 64-bit division per element)
+    const int64_t stride = int64_t(gridDim.x) * TPB;
+    int64_t idx = int64_t(blockIdx.x) * TPB + threadIdx.x;
+    auto digits = [&](int64_t x, int& d0, int& d1, int& d2, int64_t& d3) {
+        d0 = int(x % v); x /= v;
+        d1 = int(x % v); x /= v;
+        d2 = int(x % o);
+        d3 = x / o;
+    };
+    int a, b, i, sa, sb, si;
+    int64_t j, sj;
+    digits(idx, a, b, i, j);
+    digits(stride, sa, sb, si, sj);
+    // idx += stride, digit by digit (each digit sum < 2 * radix)
+    auto step = [&]() {
+        a += sa;
+        int cy = a >= v;
+        a -= cy ? int(v) : 0;
+        b += sb + cy;
+        cy = b >= v;
+        b -= cy ? int(v) : 0;
+        i += si + cy;
+        cy = i >= o;
+        i -= cy ? int(o) : 0;
+        j += sj + cy;
+    };
+    // U consecutive elements of this thread per step: all their loads are
+    // issued before any store (the arrays are distinct), then the elements
+    // are finished in order, so e and c accumulate exactly as one at a time
+    constexpr int U = 4;
+    for (; idx < n; idx += U * stride) {
+        int64_t p[U], dj[U];
+        int da[U], db[U], di[U];
+        double z2[U], t2[U], vs[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            p[k] = idx + k * stride;
+            da[k] = a;
+            db[k] = b;
+            di[k] = i;
+            dj[k] = j;
+            step();
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const bool ok = p[k] < n;
+            z2[k] = ok ? u.Z2[p[k]] : 0.0;
+            t2[k] = ok ? u.T2[p[k]] : 0.0;
+            vs[k] = ok ? u.VABIJSA[p[k]] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            if (p[k] >= n) break;
+            double den = 0.0;
+            den += u.dA[da[k]];
+            den += u.dA[db[k]];
+            den += u.dI[di[k]];
+            den += u.dI[dj[k]];
+            const double z = fabs(den) < 1e-4 ? 0.0 : z2[k] / den;
+            u.Z2[p[k]] = z;
+            const double t = t2[k] + z;
+            u.T2[p[k]] = t;
+            const double tau = t + u.T1[da[k] + di[k] * v] * u.T1[db[k] + dj[k] * v];
+            u.Tau[p[k]] = tau;
+            e += vs[k] * tau;
+            c = fmax(c, fabs(z));
+        }
     }
     e = block_reduce(e, RED_SUM);
     c = block_reduce(c, RED_MAX);
