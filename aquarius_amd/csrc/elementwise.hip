@@ -84,13 +84,20 @@ static PlanDev to_dev(const StridedPlan& p) {
 }
 
 // one thread per output element; reduction over the summed source dims
-template <class Out>
-__global__ __launch_bounds__(TPB) void k_strided(PlanDev p, const double* __restrict__ src, Out out, bool read_src) {
-    for (int64_t idx = int64_t(blockIdx.x) * TPB + threadIdx.x; idx < p.nout; idx += int64_t(gridDim.x) * TPB) {
-        int64_t r = idx, od = 0, os = 0;
+// Index type I: the flat output index and its digits (uint32_t when every
+// index and stride step stays below 2^32, which makes each digit a 32-bit
+// division); element offsets are always 64-bit.
+template <typename I, class Out>
+__device__ __forceinline__ void strided_body(const PlanDev& p, const double* __restrict__ src, Out& out, bool read_src) {
+    const I nout = I(p.nout), stride = I(gridDim.x) * I(TPB);
+    for (I idx = I(blockIdx.x) * I(TPB) + I(threadIdx.x); idx < nout; idx += stride) {
+        I r = idx;
+        int64_t od = 0, os = 0;
         for (int k = 0; k < p.nd; ++k) {
-            const int64_t c = r % p.len[k];
-            r /= p.len[k];
+            const I len = I(p.len[k]);
+            const I q = r / len;
+            const int64_t c = int64_t(r - q * len);
+            r = q;
             od += c * p.sdst[k];
             os += c * p.ssrc[k];
         }
@@ -111,6 +118,13 @@ __global__ __launch_bounds__(TPB) void k_strided(PlanDev p, const double* __rest
         }
         out.put(od, acc);
     }
+}
+
+template <class Out>
+__global__ __launch_bounds__(TPB) void k_strided(PlanDev p, const double* __restrict__ src, Out out, bool read_src) {
+    // idx + stride < 2^31 + 2^28 (grid <= 2^20 blocks of 256) never wraps 32 bits
+    if (p.nout < (int64_t(1) << 31)) strided_body<uint32_t>(p, src, out, read_src);
+    else strided_body<int64_t>(p, src, out, read_src);
 }
 
 // LDS-tiled transpose: dims d0 (unit stride in dst) and d1 (unit stride in
