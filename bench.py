@@ -125,6 +125,10 @@ def serving(a):
     cfg = aq.default_config(packed_ladder=a.packed_ladder, graph=a.graph, convergence=a.convergence,
                             diis_order=a.diis, max_iterations=150, precision=a.mixed,
                             refine_below=a.refine_below, min_reduced_flops=1e8)
+    # jobs with o <= graph_max_o replay their stage from a HIP graph (launch-bound sizes)
+    cfg_small = aq.default_config(packed_ladder=a.packed_ladder, graph=1, convergence=a.convergence,
+                                  diis_order=a.diis, max_iterations=150, precision=a.mixed,
+                                  refine_below=a.refine_below, min_reduced_flops=1e8)
     res = []
     with aq.Server(slots=a.slots, hbm_budget=int(a.budget_gb * 2**30), max_bypass_s=a.max_bypass) as srv:
         if a.grouping_us > 0:
@@ -135,7 +139,7 @@ def serving(a):
             dt = t - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            ids.append(srv.submit(seed, o, v, config=cfg))
+            ids.append(srv.submit(seed, o, v, config=cfg_small if o <= a.graph_max_o else cfg))
         for k, i in enumerate(ids):
             r = srv.wait(i)
             res.append(r)
@@ -151,7 +155,7 @@ def serving(a):
            "config": {"workload": f"{a.jobs} synthetic CCSD jobs, o~U{{{a.o_min}..{a.o_max}}}, v=round(o*U[4,8]), "
                                   f"Poisson arrivals {a.rate}/s, HBM budget {a.budget_gb} GB, {a.slots} slots",
                       "seed": a.seed, "convergence": a.convergence, "diis_order": a.diis,
-                      "graph": a.graph, "grouping_us": a.grouping_us},
+                      "graph": a.graph, "graph_max_o": a.graph_max_o, "grouping_us": a.grouping_us},
            "grouped_launches": grouping,
            "jobs": len(res), "converged": len(ok),
            "failed": sum(r["status"] == aq.JOB_FAILED for r in res),
@@ -378,6 +382,8 @@ def main():
                     help="particle-particle ladder on packed (anti)symmetric pairs (1) or as the full contraction (0)")
     ap.add_argument("--graph", type=int, default=0, help="serving: HIP-graph replay of each job's contraction stage "
                     "(stages replayed from a graph do not group)")
+    ap.add_argument("--graph-max-o", type=int, default=0,
+                    help="serving: jobs with o <= this use HIP-graph replay whatever --graph says")
     ap.add_argument("--grouping-us", type=int, default=50,
                     help="serving: cross-job grouped GEMM launches with this host window in us (0: off)")
     ap.add_argument("--compare-precisions", type=int, default=1,
