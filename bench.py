@@ -42,9 +42,11 @@ TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 87.7e9,
 
 
 def flops_per_iter(o, v):
-    """Algorithmic FLOPs of one iteration, term by term from rhfccsd.cxx:231-326."""
+    """Algorithmic FLOPs of one iteration: the sum over the 40 contractions of
+    rhfccsd.cxx:231-309 of 2 x the product of their label extents (outer
+    products included), collected by power of o and v."""
     return (2 * o**2 * v**4 + 12 * o**3 * v**3 + 14 * o**2 * v**3 + 4 * o * v**3 + 4 * o**4 * v**2
-            + 20 * o**3 * v**2 + 6 * o**2 * v**2 + 2 * o**4 * v + 4 * o**3 * v + 4 * (o**2 * v + o * v**2))
+            + 20 * o**3 * v**2 + 10 * o**2 * v**2 + 2 * o**4 * v + 4 * o**3 * v + 4 * (o**2 * v + o * v**2))
 
 
 def flops_per_iter_executed(o, v, packed):
