@@ -283,24 +283,60 @@ void ints_1e(const MoleculeData& m, double* S, double* T, double* V) {
 // ----------------------------------------------------------- two-electron
 namespace {
 
+// One primitive product of a family pair: exponent sum and centre; its
+// Hermite expansion, without contraction coefficients, at E[...] of the pair.
 struct PrimPair {
     double p, P[3];
-    size_t E;                       // offset of E[h * nab + ab] in the pair's table
+    size_t E;                       // offset of E[h * nab + ab] in the family pair's table
 };
 
-struct PairData {
-    int a = 0, b = 0, L = 0, nab = 0, nh = 0;
+// Shells of one centre with the same L and the same exponents -- the
+// contractions of one general contraction, e.g. cc-pVDZ's 1s and 2s on O --
+// at consecutive shell indices [first, first + n).  Their primitive
+// integrals are computed once and weighted per contraction.
+struct Family {
+    int first = 0, n = 0;
+};
+
+std::vector<Family> families(const MoleculeData& m) {
+    std::vector<Family> f;
+    for (int s = 0; s < int(m.shells.size()); ++s) {
+        const Shell& x = m.shells[size_t(s)];
+        if (!f.empty()) {
+            const Shell& y = m.shells[size_t(f.back().first)];
+            if (y.atom == x.atom && y.L == x.L && y.a == x.a) {
+                f.back().n += 1;
+                continue;
+            }
+        }
+        f.push_back(Family{s, 1});
+    }
+    return f;
+}
+
+// The shell pairs (a, b), a >= b, of two families F1 >= F2.  Member m's
+// integrals are the raw primitive ones weighted by coef[m * nprim + k] =
+// c_a[i] c_b[j] for primitive product k = (i, j).
+struct FamPair {
+    int L = 0, nab = 0, nh = 0;
     std::vector<PrimPair> prims;
-    std::vector<double> E;          // per primitive pair: nh x nab (includes K c_a c_b)
-    double Q = 0.0;                 // sqrt(max |(ab|ab)|)
+    std::vector<double> E;                  // per primitive product: nh x nab
+    std::vector<std::pair<int, int>> members;
+    std::vector<int64_t> pair_index;        // a(a+1)/2 + b per member
+    std::vector<double> coef;               // [member][primitive product]
+    std::vector<double> Q;                  // per member: sqrt(max |(ab|ab)| * nab)
 };
 
-PairData make_pair(const MoleculeData& m, int ia, int ib) {
-    const Shell& a = m.shells[size_t(ia)];
-    const Shell& b = m.shells[size_t(ib)];
-    PairData pd;
-    pd.a = ia;
-    pd.b = ib;
+FamPair make_fampair(const MoleculeData& m, const Family& F1, const Family& F2) {
+    const Shell& a = m.shells[size_t(F1.first)];
+    const Shell& b = m.shells[size_t(F2.first)];
+    FamPair pd;
+    for (int x = F1.first; x < F1.first + F1.n; ++x)
+        for (int y = F2.first; y < F2.first + F2.n; ++y)
+            if (x >= y) {
+                pd.members.emplace_back(x, y);
+                pd.pair_index.push_back(int64_t(x) * (x + 1) / 2 + y);
+            }
     pd.L = a.L + b.L;
     const int na = ncart(a.L), nb = ncart(b.L);
     pd.nab = na * nb;
@@ -312,11 +348,12 @@ PairData make_pair(const MoleculeData& m, int ia, int ib) {
     std::vector<double> Ex(size_t(a.L + 1) * size_t(b.L + 1) * size_t(nt)), Ey(Ex.size()), Ez(Ex.size());
     const double AB2 = (a.A[0] - b.A[0]) * (a.A[0] - b.A[0]) + (a.A[1] - b.A[1]) * (a.A[1] - b.A[1]) +
                        (a.A[2] - b.A[2]) * (a.A[2] - b.A[2]);
+    std::vector<std::pair<size_t, size_t>> ij;
     for (size_t i = 0; i < a.a.size(); ++i)
         for (size_t j = 0; j < b.a.size(); ++j) {
             const double ai = a.a[i], bj = b.a[j], p = ai + bj;
-            const double K = std::exp(-ai * bj / p * AB2) * a.c[i] * b.c[j];
-            if (std::fabs(K) < 1e-300) continue;
+            const double K = std::exp(-ai * bj / p * AB2);
+            if (K < 1e-300) continue;
             PrimPair pp;
             pp.p = p;
             for (int d = 0; d < 3; ++d) pp.P[d] = (ai * a.A[d] + bj * b.A[d]) / p;
@@ -340,16 +377,28 @@ PairData make_pair(const MoleculeData& m, int ia, int ib) {
                     }
             }
             pd.prims.push_back(pp);
+            ij.emplace_back(i, j);
         }
+    const size_t np = pd.prims.size();
+    pd.coef.resize(pd.members.size() * np);
+    for (size_t mi = 0; mi < pd.members.size(); ++mi) {
+        const Shell& sa = m.shells[size_t(pd.members[mi].first)];
+        const Shell& sb = m.shells[size_t(pd.members[mi].second)];
+        for (size_t k = 0; k < np; ++k) pd.coef[mi * np + k] = sa.c[ij[k].first] * sb.c[ij[k].second];
+    }
     return pd;
 }
 
 struct QuartetWork {
-    std::vector<double> R, work, tmp, W, out, ao, half;
+    std::vector<double> R, work, tmp, Y, W, Z, out, ao, half;
 };
 
-// Cartesian (ab|cd) into w.out[ab * ncd + cd].
-void quartet(const PairData& bra, const PairData& ket, QuartetWork& w) {
+// Cartesian (ab|cd) of every bra member mb and ket member mk into
+// w.out[(mb * nmk + mk) * nab * ncd + ab * ncd + cd].  The Hermite integrals
+// and their product with the ket expansion are formed once per primitive
+// quartet and weighted per member; a family of one folds its weight into the
+// accumulation instead.
+void quartet(const FamPair& bra, const FamPair& ket, QuartetWork& w) {
     const int Lt = bra.L + ket.L;
     const size_t D = size_t(Lt + 1);
     w.R.resize(D * D * D);
@@ -357,12 +406,19 @@ void quartet(const PairData& bra, const PairData& ket, QuartetWork& w) {
     const Herm& Hb = hermite_list(bra.L);
     const Herm& Hk = hermite_list(ket.L);
     const int nhb = bra.nh, nhk = ket.nh, nab = bra.nab, ncd = ket.nab;
+    const size_t nmb = bra.members.size(), nmk = ket.members.size();
+    const size_t npb = bra.prims.size(), npk = ket.prims.size();
+    const size_t blk = size_t(nab) * size_t(ncd), wsz = size_t(nhb) * size_t(ncd);
     w.tmp.resize(size_t(nhb) * size_t(nhk));
-    w.W.resize(size_t(nhb) * size_t(ncd));
-    w.out.assign(size_t(nab) * size_t(ncd), 0.0);
-    for (const PrimPair& p1 : bra.prims) {
+    w.Y.resize(wsz);
+    w.W.resize(wsz * nmk);
+    w.Z.resize(blk);
+    w.out.assign(blk * nmb * nmk, 0.0);
+    for (size_t k1 = 0; k1 < npb; ++k1) {
+        const PrimPair& p1 = bra.prims[k1];
         std::fill(w.W.begin(), w.W.end(), 0.0);
-        for (const PrimPair& p2 : ket.prims) {
+        for (size_t k2 = 0; k2 < npk; ++k2) {
+            const PrimPair& p2 = ket.prims[k2];
             const double p = p1.p, q = p2.p;
             const double alpha = p * q / (p + q);
             const double pref = 2.0 * std::pow(M_PI, 2.5) / (p * q * std::sqrt(p + q));
@@ -377,26 +433,49 @@ void quartet(const PairData& bra, const PairData& ket, QuartetWork& w) {
                 }
             }
             const double* E2 = ket.E.data() + p2.E;
+            // one ket member: W += (c * tmp) x E2; several: Y = tmp x E2, W_m += c_m Y
+            const double c2 = nmk == 1 ? ket.coef[k2] : 1.0;
+            double* acc = nmk == 1 ? w.W.data() : w.Y.data();
+            if (nmk > 1) std::fill(w.Y.begin(), w.Y.end(), 0.0);
             for (int hb = 0; hb < nhb; ++hb) {
-                double* Wr = w.W.data() + size_t(hb) * size_t(ncd);
+                double* Wr = acc + size_t(hb) * size_t(ncd);
                 const double* tr = w.tmp.data() + size_t(hb) * size_t(nhk);
                 for (int hk = 0; hk < nhk; ++hk) {
-                    const double t = tr[hk];
+                    const double t = tr[hk] * c2;
                     const double* er = E2 + size_t(hk) * size_t(ncd);
                     for (int cd = 0; cd < ncd; ++cd) Wr[cd] += t * er[cd];
                 }
             }
+            if (nmk > 1)
+                for (size_t mk = 0; mk < nmk; ++mk) {
+                    const double c = ket.coef[mk * npk + k2];
+                    double* Wm = w.W.data() + mk * wsz;
+                    for (size_t x = 0; x < wsz; ++x) Wm[x] += c * w.Y[x];
+                }
         }
+        // out(mb, mk) += c_mb(k1) * E1^T W_mk
         const double* E1 = bra.E.data() + p1.E;
-        for (int hb = 0; hb < nhb; ++hb) {
-            const double* er = E1 + size_t(hb) * size_t(nab);
-            const double* Wr = w.W.data() + size_t(hb) * size_t(ncd);
-            for (int ab = 0; ab < nab; ++ab) {
-                const double e = er[ab];
-                if (e == 0.0) continue;
-                double* o = w.out.data() + size_t(ab) * size_t(ncd);
-                for (int cd = 0; cd < ncd; ++cd) o[cd] += e * Wr[cd];
+        for (size_t mk = 0; mk < nmk; ++mk) {
+            const double* Wm = w.W.data() + mk * wsz;
+            const double c1 = nmb == 1 ? bra.coef[k1] : 1.0;
+            double* acc = nmb == 1 ? w.out.data() + mk * blk : w.Z.data();
+            if (nmb > 1) std::fill(w.Z.begin(), w.Z.end(), 0.0);
+            for (int hb = 0; hb < nhb; ++hb) {
+                const double* er = E1 + size_t(hb) * size_t(nab);
+                const double* Wr = Wm + size_t(hb) * size_t(ncd);
+                for (int ab = 0; ab < nab; ++ab) {
+                    if (er[ab] == 0.0) continue;
+                    const double e = er[ab] * c1;
+                    double* o = acc + size_t(ab) * size_t(ncd);
+                    for (int cd = 0; cd < ncd; ++cd) o[cd] += e * Wr[cd];
+                }
             }
+            if (nmb > 1)
+                for (size_t mb = 0; mb < nmb; ++mb) {
+                    const double c = bra.coef[mb * npb + k1];
+                    double* o = w.out.data() + (mb * nmk + mk) * blk;
+                    for (size_t x = 0; x < blk; ++x) o[x] += c * w.Z[x];
+                }
         }
     }
 }
@@ -422,19 +501,29 @@ void transform_index(std::vector<double>& x, int dims[4], int k, const Shell& s,
     dims[k] = nf;
 }
 
+// shell pair index a(a+1)/2 + b (a >= b) -> (a, b)
+void unpair(int64_t idx, int& a, int& b) {
+    int64_t x = int64_t((std::sqrt(8.0 * double(idx) + 1.0) - 1.0) / 2.0);
+    while (x * (x + 1) / 2 > idx) --x;
+    while ((x + 1) * (x + 2) / 2 <= idx) ++x;
+    a = int(x);
+    b = int(idx - x * (x + 1) / 2);
+}
+
 }  // namespace
 
-// Every canonical shell quartet (bra pair i >= ket pair j, each pair a >= b)
-// evaluated once, its pure-harmonic block [fa][fb][fc][fd] handed to
-// sink(i, j, block) (block == nullptr: screened to zero).  Runs `nthreads`
-// workers over bra pairs in descending cost; distinct (i, j) are distinct calls.
+// Every canonical shell quartet (bra pair i >= ket pair j, each pair a >= b,
+// i and j the pair indices a(a+1)/2 + b) evaluated once, its pure-harmonic
+// block [fa][fb][fc][fd] handed to sink(i, j, block) (block == nullptr:
+// screened to zero).  The work runs over pairs of shell families (the
+// contractions of a general contraction share their primitive integrals),
+// on `nthreads` workers; distinct (i, j) are distinct calls.
 template <class Sink>
-static void for_each_quartet(const MoleculeData& m, int nthreads, const std::vector<PairData>*& out_pairs,
-                             std::vector<PairData>& pairs, Sink&& sink) {
-    const int ns = int(m.shells.size());
-    const size_t npair = size_t(ns) * size_t(ns + 1) / 2;
-    pairs.assign(npair, PairData());
-    out_pairs = &pairs;
+static void for_each_quartet(const MoleculeData& m, int nthreads, Sink&& sink) {
+    const std::vector<Family> fam = families(m);
+    const int64_t nfam = int64_t(fam.size());
+    const size_t nfp = size_t(nfam) * size_t(nfam + 1) / 2;
+    std::vector<FamPair> fps(nfp);
     if (nthreads <= 0) nthreads = int(std::max(1u, std::thread::hardware_concurrency()));
     nthreads = std::min(nthreads, 64);
 
@@ -460,63 +549,88 @@ static void for_each_quartet(const MoleculeData& m, int nthreads, const std::vec
         if (err) std::rethrow_exception(err);
     };
 
-    // shell pairs (a >= b) and their Schwarz factors
+    // family pairs (F1 >= F2) and the Schwarz factor of each member shell pair
     run(
         [&](size_t idx, QuartetWork& w) {
-            int a = int((std::sqrt(8.0 * double(idx) + 1.0) - 1.0) / 2.0);
-            while (size_t(a) * size_t(a + 1) / 2 > idx) --a;
-            while (size_t(a + 1) * size_t(a + 2) / 2 <= idx) ++a;
-            const int b = int(idx - size_t(a) * size_t(a + 1) / 2);
-            pairs[idx] = make_pair(m, a, b);
-            quartet(pairs[idx], pairs[idx], w);
-            const int nab = pairs[idx].nab;
-            double mx = 0.0;
-            for (int i = 0; i < nab; ++i) mx = std::max(mx, std::fabs(w.out[size_t(i) * size_t(nab) + size_t(i)]));
-            // the cartesian diagonal bounds every pure-harmonic combination up to
-            // the (normalised) transform's row norm, which is <= sqrt(ncart)
-            pairs[idx].Q = std::sqrt(mx * nab);
-        },
-        npair);
-
-    const double cutoff = 1e-15;
-    // bra pairs processed in descending cost order
-    run(
-        [&](size_t r, QuartetWork& w) {
-            const size_t i = npair - 1 - r;
-            const PairData& bra = pairs[i];
-            const Shell& sa = m.shells[size_t(bra.a)];
-            const Shell& sb = m.shells[size_t(bra.b)];
-            std::vector<double> tmp;
-            for (size_t j = 0; j <= i; ++j) {
-                const PairData& ket = pairs[j];
-                const Shell& sc = m.shells[size_t(ket.a)];
-                const Shell& sd = m.shells[size_t(ket.b)];
-                int dims[4] = {ncart(sa.L), ncart(sb.L), ncart(sc.L), ncart(sd.L)};
-                if (bra.Q * ket.Q < cutoff) {
-                    sink(i, j, static_cast<const double*>(nullptr));
-                    continue;
-                }
-                quartet(bra, ket, w);
-                w.ao.swap(w.out);
-                transform_index(w.ao, dims, 0, sa, tmp);
-                transform_index(w.ao, dims, 1, sb, tmp);
-                transform_index(w.ao, dims, 2, sc, tmp);
-                transform_index(w.ao, dims, 3, sd, tmp);
-                sink(i, j, static_cast<const double*>(w.ao.data()));
+            int f1, f2;
+            unpair(int64_t(idx), f1, f2);
+            FamPair& fp = fps[idx];
+            fp = make_fampair(m, fam[size_t(f1)], fam[size_t(f2)]);
+            quartet(fp, fp, w);
+            const size_t nm = fp.members.size(), blk = size_t(fp.nab) * size_t(fp.nab);
+            fp.Q.resize(nm);
+            for (size_t mi = 0; mi < nm; ++mi) {
+                const double* o = w.out.data() + (mi * nm + mi) * blk;
+                double mx = 0.0;
+                for (int i = 0; i < fp.nab; ++i) mx = std::max(mx, std::fabs(o[size_t(i) * size_t(fp.nab) + size_t(i)]));
+                // the cartesian diagonal bounds every pure-harmonic combination up to
+                // the (normalised) transform's row norm, which is <= sqrt(ncart)
+                fp.Q[mi] = std::sqrt(mx * fp.nab);
             }
         },
-        npair);
+        nfp);
+
+    const double cutoff = 1e-15;
+    // bra family pairs from the last (largest shell indices) down
+    run(
+        [&](size_t r, QuartetWork& w) {
+            const size_t I = nfp - 1 - r;
+            const FamPair& bra = fps[I];
+            const double qb = *std::max_element(bra.Q.begin(), bra.Q.end());
+            std::vector<double> tmp;
+            for (size_t J = 0; J <= I; ++J) {
+                const FamPair& ket = fps[J];
+                const double qk = *std::max_element(ket.Q.begin(), ket.Q.end());
+                const bool any = qb * qk >= cutoff;
+                if (any) quartet(bra, ket, w);
+                const size_t nmb = bra.members.size(), nmk = ket.members.size();
+                const size_t blk = size_t(bra.nab) * size_t(ket.nab);
+                for (size_t mb = 0; mb < nmb; ++mb)
+                    for (size_t mk = 0; mk < nmk; ++mk) {
+                        const int64_t i = bra.pair_index[mb], j = ket.pair_index[mk];
+                        if (I == J && i < j) continue;   // the same quartet as (mk, mb)
+                        const int64_t hi = std::max(i, j), lo = std::min(i, j);
+                        if (!any || bra.Q[mb] * ket.Q[mk] < cutoff) {
+                            sink(size_t(hi), size_t(lo), static_cast<const double*>(nullptr));
+                            continue;
+                        }
+                        const Shell& sa = m.shells[size_t(bra.members[mb].first)];
+                        const Shell& sb = m.shells[size_t(bra.members[mb].second)];
+                        const Shell& sc = m.shells[size_t(ket.members[mk].first)];
+                        const Shell& sd = m.shells[size_t(ket.members[mk].second)];
+                        int dims[4] = {ncart(sa.L), ncart(sb.L), ncart(sc.L), ncart(sd.L)};
+                        const double* o = w.out.data() + (mb * nmk + mk) * blk;
+                        w.ao.assign(o, o + blk);
+                        transform_index(w.ao, dims, 0, sa, tmp);
+                        transform_index(w.ao, dims, 1, sb, tmp);
+                        transform_index(w.ao, dims, 2, sc, tmp);
+                        transform_index(w.ao, dims, 3, sd, tmp);
+                        if (i >= j) {
+                            sink(size_t(i), size_t(j), static_cast<const double*>(w.ao.data()));
+                        } else {
+                            // bra and ket exchanged: [fc][fd][fa][fb]
+                            const size_t nb2 = size_t(dims[0]) * size_t(dims[1]), nk2 = size_t(dims[2]) * size_t(dims[3]);
+                            w.half.resize(nb2 * nk2);
+                            for (size_t x = 0; x < nb2; ++x)
+                                for (size_t y = 0; y < nk2; ++y) w.half[y * nb2 + x] = w.ao[x * nk2 + y];
+                            sink(size_t(j), size_t(i), static_cast<const double*>(w.half.data()));
+                        }
+                    }
+            }
+        },
+        nfp);
 }
 
 void ints_2e(const MoleculeData& m, int nthreads, double* eri) {
     const size_t N = size_t(m.nao);
-    std::vector<PairData> pairs;
-    const std::vector<PairData>* P = nullptr;
-    for_each_quartet(m, nthreads, P, pairs, [&](size_t i, size_t j, const double* blk) {
-        const Shell& sa = m.shells[size_t((*P)[i].a)];
-        const Shell& sb = m.shells[size_t((*P)[i].b)];
-        const Shell& sc = m.shells[size_t((*P)[j].a)];
-        const Shell& sd = m.shells[size_t((*P)[j].b)];
+    for_each_quartet(m, nthreads, [&](size_t i, size_t j, const double* blk) {
+        int ia, ib, ic, id;
+        unpair(int64_t(i), ia, ib);
+        unpair(int64_t(j), ic, id);
+        const Shell& sa = m.shells[size_t(ia)];
+        const Shell& sb = m.shells[size_t(ib)];
+        const Shell& sc = m.shells[size_t(ic)];
+        const Shell& sd = m.shells[size_t(id)];
         const int d0 = sa.nf, d1 = sb.nf, d2 = sc.nf, d3 = sd.nf;
         for (int f0 = 0; f0 < d0; ++f0)
             for (int f1 = 0; f1 < d1; ++f1)
@@ -576,9 +690,7 @@ CompactEri compact_layout(const MoleculeData& m) {
 }
 
 void ints_2e_compact(const MoleculeData& m, int nthreads, const CompactEri& c, double* out) {
-    std::vector<PairData> pairs;
-    const std::vector<PairData>* P = nullptr;
-    for_each_quartet(m, nthreads, P, pairs, [&](size_t i, size_t j, const double* blk) {
+    for_each_quartet(m, nthreads, [&](size_t i, size_t j, const double* blk) {
         const size_t n = size_t(c.pair_size[i]) * size_t(c.pair_size[j]);
         double* dst = out + c.rowstart[i] + c.pair_size[i] * c.prefix[j];
         if (blk) std::memcpy(dst, blk, n * sizeof(double));
