@@ -10,9 +10,11 @@
 //   (ab|cd) = 2 pi^{5/2} / (p q sqrt(p+q)) sum_{tuv} E^{ab}_{tuv}
 //             sum_{t'u'v'} (-1)^{t'+u'+v'} E^{cd}_{t'u'v'} R_{t+t',u+u',v+v'}.
 // Shell quartets with a Cauchy-Schwarz bound below 1e-15 (the reference's
-// calc_cutoff, 2eints.cxx:524-531) are written as zeros.  Quartets are spread
-// over threads by canonical bra pair; distinct canonical quartets own
-// disjoint output elements, so the threads never write the same address.
+// calc_cutoff, 2eints.cxx:524-531) are written as zeros.  The contractions of
+// one general contraction (same centre, L and exponents) share their
+// primitive quartets.  Work is spread over threads by canonical bra pair of
+// shell families; distinct canonical quartets own disjoint output elements,
+// so the threads never write the same address.
 #include "frontend.hpp"
 #include "internal.hpp"
 
