@@ -6,14 +6,20 @@ with the size-normalised factor scale), integrals and amplitudes resident in
 HBM (the full 55 GB FP64 vvvv block included).
 
 A "step" is one CCSD iteration (RHFCCSD::iterate: intermediates, residual,
-update, energy, DIIS).  The FP64 solve runs to convergence first (the
-reference arithmetic); the mixed solve (BF16x3 MFMA contractions with FP32
-accumulation until the residual drops below 1e-5, FP64 afterwards) then runs
-W untimed warmup iterations and K timed iterations.  K = 0 (the default)
-times every remaining iteration of the mixed solve up to convergence, so
-`value` is the whole solve's iterations/s, FP64 refinement included; an
-explicit K times exactly K iterations (the solve then continues untimed to
-convergence so its energy can be compared with the FP64 one).
+update, energy, DIIS).  After W untimed iterations in each arithmetic on a
+throwaway solver (GEMM tuner trials, operand copies of the integrals), the
+FP64 solve runs to convergence (the reference arithmetic), then the mixed
+solve (BF16x3 MFMA contractions with FP32 accumulation until the residual
+drops below 1e-5, FP64 refinement afterwards) runs to convergence.  `value`
+is the mixed solve's iterations/s over every iteration of that solve, FP64
+refinement included, because the metric is defined at the converged energy;
+`steps` is the number of iterations timed.  The BF16x3 phase's own rate is a
+separate field.
+
+The same run then solves the CPU leg's system (o=20, v=140) in FP64 on the
+GPU, serves the 256-job stream (continuous batching; jobs/h, p50/p99
+time-to-E and service time, one-at-a-time FP64 baseline, per-job |dE|) and
+times the oracle on the host's cores on that o=20, v=140 system.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this engine runs on one
 GPU (sharding over GPUs would split Z2's output pair index; it is not built).
@@ -83,28 +89,60 @@ def roofline_of(timing, prefix, peak):
     return out
 
 
-def cpu_baseline(o_s, v_s, iters, seed, o, v):
-    """Oracle (numpy/OpenBLAS) iterations on a smaller system of the same
-    construction, scaled to the workload by the FLOP model."""
-    import numpy as np  # noqa: F401
+def host_cpu():
+    """Host cores this process may use and the CPU model name."""
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return cores, model
+
+
+def cpu_baseline(seed, o, v, nq, iters, gpu_fp64_rate, o_big, v_big):
+    """The oracle (numpy + the host BLAS) timed on this host's cores: `iters`
+    iterations of the given synthetic system, each timed with a wall clock
+    around RHFCCSD.iterate as Iterative::run does (iterative.hpp:134-141),
+    after one untimed iteration (BLAS threads, page faults).  Setup (MP2
+    guess) and the integral build are outside the timed region.  The larger
+    workload's rate is a FLOP-ratio estimate, labelled as such."""
+    import numpy as np
     from oracle import rhfccsd, synth
     try:
         from threadpoolctl import threadpool_info
         threads = max(int(i.get("num_threads", 1)) for i in threadpool_info()) or 1
     except Exception:
         threads = os.cpu_count() or 1
-    f, V, _, _ = synth.synthetic_system(seed, o_s, v_s)
+    cores, model = host_cpu()
+    f, V, _, _ = synth.synthetic_system(seed, o, v, nq)
     cc = rhfccsd.RHFCCSD(f, V)
-    cc.iterate()   # warm (BLAS threads, page faults)
-    t0 = time.perf_counter()
+    cc.iterate()
+    per = []
     for _ in range(iters):
+        t0 = time.perf_counter()
         cc.iterate()
-    dt = (time.perf_counter() - t0) / iters
-    scale = flops_per_iter(o_s, v_s) / flops_per_iter(o, v)
-    return {"value": scale / dt, "unit": "iter/s", "cores": threads, "kind": "port",
-            "sample": f"{iters} oracle iterations of the o={o_s}, v={v_s} synthetic system "
-                      f"({dt:.2f} s/iter, numpy+OpenBLAS on {threads} threads), scaled to o={o}, v={v} by the "
-                      f"FLOP ratio F({o_s},{v_s})/F({o},{v}) = {scale:.3e}"}
+        per.append(time.perf_counter() - t0)
+    s_it = float(np.median(per))
+    out = {"value": 1.0 / s_it, "unit": "iter/s", "cores": threads, "kind": "port",
+           "sample": f"{iters} timed iterations (after 1 untimed) of the oracle's RHF-CCSD on the synthetic system "
+                     f"o={o}, v={v}, nQ={nq}, seed {seed}, FP64, numpy + host BLAS on {threads} threads; "
+                     f"wall clock per iterate, median {s_it:.2f} s/iteration",
+           "host": {"cpu_model": model, "cores_available": cores, "blas_threads": threads},
+           "s_per_iter": per, "gpu_fp64_iter_per_s_same_system": gpu_fp64_rate,
+           "gpu_fp64_vs_cpu": (gpu_fp64_rate * s_it) if gpu_fp64_rate else None}
+    scale = flops_per_iter(o, v) / flops_per_iter(o_big, v_big)
+    out["estimate_at_bench_workload"] = {
+        "value": scale / s_it, "unit": "iter/s", "o": o_big, "v": v_big,
+        "how": f"estimate, not measured: the rate above scaled by F({o},{v})/F({o_big},{v_big}) = {scale:.3e}"}
+    return out
 
 
 def job_stream(seed, n, o_min, o_max, rate):
@@ -119,11 +157,16 @@ def job_stream(seed, n, o_min, o_max, rate):
     return [(int(1000 + i), int(o[i]), int(v[i]), float(t[i])) for i in range(n)]
 
 
-def serving(a):
-    """Continuous-batching serving run; prints one JSON line."""
+def serve_stream(a, seed):
+    """Continuous-batching serving run of the job stream; returns a dict.
+
+    Time-to-E is submit -> converged energy (queueing included); service time
+    is start -> converged energy on the device.  A sample of the same jobs is
+    then solved one at a time in FP64 (no arrivals: back to back), which gives
+    the one-at-a-time baseline and each sampled job's |dE| mixed vs FP64."""
     import numpy as np
     import aquarius_amd as aq
-    jobs = job_stream(a.seed, a.jobs, a.o_min, a.o_max, a.rate)
+    jobs = job_stream(seed, a.jobs, a.o_min, a.o_max, a.rate)
     cfg = aq.default_config(packed_ladder=a.packed_ladder, graph=a.graph, convergence=a.convergence,
                             diis_order=a.diis, max_iterations=150, precision=a.mixed,
                             refine_below=a.refine_below, min_reduced_flops=1e8)
@@ -137,11 +180,11 @@ def serving(a):
             srv.set_grouping(a.grouping_us)
         t0 = time.perf_counter()
         ids = []
-        for seed, o, v, t in jobs:
+        for jseed, o, v, t in jobs:
             dt = t - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            ids.append(srv.submit(seed, o, v, config=cfg_small if o <= a.graph_max_o else cfg))
+            ids.append(srv.submit(jseed, o, v, config=cfg_small if o <= a.graph_max_o else cfg))
         for k, i in enumerate(ids):
             r = srv.wait(i)
             res.append(r)
@@ -152,34 +195,46 @@ def serving(a):
         grouping = srv.grouping_stats()
     ok = [r for r in res if r["status"] == aq.JOB_DONE and r["converged"]]
     tte = np.array([r["t_end"] - r["t_submit"] for r in ok])
+    svc = np.array([r["t_end"] - r["t_start"] for r in ok])
+
+    def pct(x):
+        return {"p50": float(np.percentile(x, 50)), "p99": float(np.percentile(x, 99)),
+                "max": float(x.max()), "mean": float(x.mean())} if len(x) else None
+
     out = {"metric": "served jobs/h + p99 time-to-E", "value": 3600.0 * len(ok) / wall, "unit": "jobs/h",
            "n_gpus": 1, "higher_is_better": True, "dtype": f"{a.mixed}+fp64", "data": "synthetic",
            "config": {"workload": f"{a.jobs} synthetic CCSD jobs, o~U{{{a.o_min}..{a.o_max}}}, v=round(o*U[4,8]), "
                                   f"Poisson arrivals {a.rate}/s, HBM budget {a.budget_gb} GB, {a.slots} slots",
-                      "seed": a.seed, "convergence": a.convergence, "diis_order": a.diis,
+                      "seed": seed, "convergence": a.convergence, "diis_order": a.diis,
                       "graph": a.graph, "graph_max_o": a.graph_max_o, "grouping_us": a.grouping_us},
            "grouped_launches": grouping,
+           "gemms_per_grouped_launch": (grouping[1] / grouping[0]) if grouping[0] else None,
            "jobs": len(res), "converged": len(ok),
            "failed": sum(r["status"] == aq.JOB_FAILED for r in res),
            "first_error": next((r["error"] for r in res if r["status"] == aq.JOB_FAILED), ""),
-           "wall_s": wall,
-           "time_to_E_s": {"p50": float(np.percentile(tte, 50)), "p99": float(np.percentile(tte, 99)),
-                           "max": float(tte.max()), "mean": float(tte.mean())}}
-    # one-at-a-time FP64 solves of a sample of the same stream (no arrivals: back to back)
+           "wall_s": wall, "time_to_E_s": pct(tte), "service_time_s": pct(svc)}
     if a.fp64_sample:
-        sample = jobs[::max(1, len(jobs) // a.fp64_sample)][:a.fp64_sample]
-        c64 = aq.default_config(packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=150)
-        times = []
+        sample_idx = list(range(0, len(jobs), max(1, len(jobs) // a.fp64_sample)))[:a.fp64_sample]
+        c64 = aq.default_config(packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis,
+                                max_iterations=150)
+        times, dE = [], []
         with aq.Server(slots=1, hbm_budget=int(a.budget_gb * 2**30)) as srv:
-            for seed, o, v, _ in sample:
-                r = srv.wait(srv.submit(seed, o, v, config=c64))
+            for k in sample_idx:
+                jseed, o, v, _ = jobs[k]
+                r = srv.wait(srv.submit(jseed, o, v, config=c64))
                 times.append(r["t_end"] - r["t_start"])
-        mix_times = {jobs[k][0]: res[k]["t_end"] - res[k]["t_start"] for k in range(len(jobs))}
-        out["fp64_one_at_a_time"] = {"sample_jobs": len(sample), "mean_solve_s": float(np.mean(times)),
-                                     "jobs_per_h": 3600.0 / float(np.mean(times)),
-                                     "mixed_mean_solve_s_same_jobs_in_mix": float(np.mean([mix_times[s[0]] for s in sample]))}
+                if res[k]["status"] == aq.JOB_DONE:
+                    dE.append({"o": o, "v": v, "dE": res[k]["energy"] - r["energy"]})
+        out["fp64_one_at_a_time"] = {
+            "sample_jobs": len(sample_idx), "mean_solve_s": float(np.mean(times)),
+            "jobs_per_h": 3600.0 / float(np.mean(times)),
+            "mixed_mean_service_s_same_jobs_in_mix": float(np.mean([svc_k for svc_k in
+                                                                   (res[k]["t_end"] - res[k]["t_start"]
+                                                                    for k in sample_idx)]))}
         out["vs_fp64_one_at_a_time"] = out["value"] / out["fp64_one_at_a_time"]["jobs_per_h"]
-    print(json.dumps(out))
+        out["dE_mixed_vs_fp64_per_sampled_job"] = dE
+        out["max_abs_dE_mixed_vs_fp64"] = max((abs(d["dE"]) for d in dE), default=None)
+    return out
 
 
 def warm_tuner(a, ctx, ints):
@@ -187,13 +242,12 @@ def warm_tuner(a, ctx, ints):
     first-encounter trials (and the operand caches' first fills) fall outside
     every timed solve."""
     import aquarius_amd as aq
-    if not a.autotune:
-        return
     t0 = time.perf_counter()
     for prec in ("fp64", a.mixed):
-        cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=1,
-                        precision=prec, refine_below=a.refine_below, min_reduced_flops=1e8)
-        cc.iterate()
+        cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis,
+                        max_iterations=1000, precision=prec, refine_below=a.refine_below, min_reduced_flops=1e8)
+        for _ in range(max(1, a.warmup)):
+            cc.iterate()
         cc.close()
     ctx.trim()
     log(f"GEMM tuner warmed in {time.perf_counter() - t0:.1f}s ({len(aq.autotune_table())} shape buckets)")
@@ -348,12 +402,48 @@ def molecular(a):
     print(json.dumps(out))
 
 
+def timed_solve(aq, ctx, ints, a, prec, label):
+    """One solve to convergence, every iteration timed with a wall clock
+    around RHFCCSD::iterate (iterative.hpp:134-141; solver setup outside).
+    Returns the solver (open) and the per-iteration history."""
+    cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis,
+                    max_iterations=200, precision=prec, refine_below=a.refine_below, min_reduced_flops=1e8)
+    hist = []
+    while True:
+        p = cc.current_precision()
+        t1 = time.perf_counter()
+        e, c = cc.iterate()
+        dt = time.perf_counter() - t1
+        hist.append({"energy": e, "conv": c, "precision": p, "seconds": dt})
+        log(f"{label} it {len(hist):2d} [{['fp64', 'fp32', 'bf16x3'][p]}] E={e:.12f} conv={c:.2e} {dt * 1e3:.0f} ms")
+        if c < a.convergence or len(hist) >= 200:
+            break
+    return cc, hist
+
+
+def solve_summary(hist, convergence):
+    n = len(hist)
+    t = sum(h["seconds"] for h in hist)
+    phases = {}
+    for h in hist:
+        ph = phases.setdefault(["fp64", "fp32", "bf16x3"][h["precision"]], {"iterations": 0, "seconds": 0.0})
+        ph["iterations"] += 1
+        ph["seconds"] += h["seconds"]
+    for ph in phases.values():
+        ph["iter_per_s"] = ph["iterations"] / ph["seconds"]
+        ph["ms_per_iter"] = 1e3 * ph["seconds"] / ph["iterations"]
+    return {"iterations": n, "converged": hist[-1]["conv"] < convergence, "energy": hist[-1]["energy"],
+            "time_to_E_s": t, "iter_per_s": n / t, "ms_per_iter": 1e3 * t / n, "phases": phases}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--molecule", default=None, help="h2o | h2o-bohr | waterN: molecular workload via the "
                                                      "integral front end instead of the synthetic system")
     ap.add_argument("--basis", default="cc-pVDZ")
-    ap.add_argument("--serving", action="store_true", help="run the continuous-batching serving workload")
+    ap.add_argument("--serving", action="store_true", help="run only the continuous-batching serving workload")
+    ap.add_argument("--serving-in-line", type=int, default=1,
+                    help="default run: also run the serving stream and report it as the line's `serving` object")
     ap.add_argument("--jobs", type=int, default=256)
     ap.add_argument("--o-min", type=int, default=4)
     ap.add_argument("--o-max", type=int, default=32)
@@ -361,10 +451,15 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=240.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-bypass", type=float, default=30.0)
-    ap.add_argument("--fp64-sample", type=int, default=32)
+    ap.add_argument("--fp64-sample", type=int, default=16)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0, help="timed iterations (0: the rest of the mixed solve)")
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0,
+                    help="kept for the harness; the timed region is always the complete mixed solve (every "
+                         "iteration to convergence, FP64 refinement included), because the metric is defined at "
+                         "the converged energy")
+    ap.add_argument("--warmup", type=int, default=1,
+                    help="untimed iterations in each arithmetic on a throwaway solver before the timed solves "
+                         "(GEMM tuner trials, operand copies of the integrals)")
     ap.add_argument("--o", type=int, default=48)
     ap.add_argument("--v", type=int, default=288)
     ap.add_argument("--nq", type=int, default=0)
@@ -373,7 +468,8 @@ def main():
     ap.add_argument("--diis", type=int, default=8)
     ap.add_argument("--mixed", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--refine-below", type=float, default=1e-5)
-    ap.add_argument("--cpu-sample", default="16,112,2", help="o,v,iterations of the CPU oracle sample ('' to skip)")
+    ap.add_argument("--cpu-sample", default="1,20,140,2",
+                    help="seed,o,v,iterations of the timed CPU oracle leg ('' to skip)")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solve")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="profiling run: W warmup + K timed iterations of one solve, then exit")
@@ -388,7 +484,7 @@ def main():
                     help="serving: jobs with o <= this use HIP-graph replay whatever --graph says")
     ap.add_argument("--grouping-us", type=int, default=50,
                     help="serving: cross-job grouped GEMM launches with this host window in us (0: off)")
-    ap.add_argument("--compare-precisions", type=int, default=1,
+    ap.add_argument("--compare-precisions", type=int, default=0,
                     help="also solve with the other reduced arithmetic (FP32 MFMA beside BF16x3) and report its "
                          "energy and T2 error against FP64")
     a = ap.parse_args()
@@ -398,19 +494,19 @@ def main():
                           "not built", "n_gpus": a.gpus}))
         sys.exit(2)
 
+    serving_seed = a.seed if (a.seed is not None and a.serving) else 4
     if a.seed is None:
         a.seed = 7 if a.molecule else (4 if a.serving else 3)
     if a.molecule:
         return molecular(a)
     if a.serving:
-        serving(a)
+        print(json.dumps(serve_stream(a, serving_seed)))
         return
     if a.copy_calibration:
         return copy_calibration()
     if a.only:
         return only(a)
 
-    import numpy as np
     import aquarius_amd as aq
 
     o, v = a.o, a.v
@@ -423,9 +519,7 @@ def main():
     ctx.sync()
     log(f"integrals o={o} v={v} nQ={nq} built in {time.perf_counter() - t0:.1f}s; "
         f"HBM in use {ctx.memory()[0] / 2**30:.1f} GiB")
-
     warm_tuner(a, ctx, ints)
-    out = {}
     t2_64 = None
 
     def max_dt2(cc):
@@ -438,105 +532,62 @@ def main():
         d.free()
         return r
 
-    # ---- FP64 reference solve (to convergence)
-    e64 = None
+    # ---- FP64 solve to convergence (the reference arithmetic)
+    s64 = None
     if not a.no_fp64:
-        cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=150)
         ctx.enable_timing(True)
         ctx.timing_reset()
-        t0 = time.perf_counter()
-        r64 = cc.run(callback=lambda h: log(f"fp64 it {h['iter']:2d} E={h['energy']:.12f} conv={h['conv']:.2e} "
-                                              f"{h['seconds'] * 1e3:.0f} ms"))
-        t64 = time.perf_counter() - t0
+        cc, h64 = timed_solve(aq, ctx, ints, a, "fp64", "fp64")
         tim64 = ctx.timing()
-        e64 = r64["energy"]
-        its = [h["seconds"] for h in cc.history]
-        steady = its[1:] if len(its) > 2 else its
-        out["fp64"] = {"iterations": r64["iterations"], "converged": r64["converged"], "energy": e64,
-                       "time_to_E_s": t64, "iter_per_s": len(steady) / sum(steady),
-                       "ms_per_iter_median": 1e3 * float(np.median(steady)),
-                       "roofline": roofline_of(tim64, "gemm_f64", PEAK["fp64"])}
+        s64 = solve_summary(h64, a.convergence)
+        s64["roofline"] = roofline_of(tim64, "gemm_f64", PEAK["fp64"])
         t2_64 = cc.tensor("T2").copy()
         cc.close()
 
-    # ---- mixed solve: W warmup, exactly K timed iterations, then on to convergence
-    cc = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
-                    precision=a.mixed, refine_below=a.refine_below, min_reduced_flops=1e8)
+    # ---- mixed solve to convergence: the timed region of the headline value
     ctx.enable_timing(True)
     ctx.timing_reset()
-    hist = []
-    tstart = time.perf_counter()
-    for _ in range(a.warmup):
-        p = cc.current_precision()
-        e, c = cc.iterate()
-        hist.append((e, c, p))
-    ctx.timing_reset()
     ctx.sync()
-    t0 = time.perf_counter()
-    steps = 0
-    phase_t = {}
-    while (a.steps > 0 and steps < a.steps) or (a.steps == 0 and hist[-1][1] >= a.convergence and len(hist) < 200):
-        p = cc.current_precision()
-        t1 = time.perf_counter()
-        e, c = cc.iterate()
-        dt = time.perf_counter() - t1
-        hist.append((e, c, p))
-        steps += 1
-        phase_t.setdefault(p, []).append(dt)
-        log(f"mixed it {len(hist):2d} [{['fp64', 'fp32', 'bf16x3'][p]}] E={e:.12f} conv={c:.2e} {dt * 1e3:.0f} ms")
+    tw0 = time.perf_counter()
+    cc, hm = timed_solve(aq, ctx, ints, a, a.mixed, "mixed")
     ctx.sync()
-    t_timed = time.perf_counter() - t0
+    t_window = time.perf_counter() - tw0
     timm = ctx.timing()
-    n_conv = next((i + 1 for i, h in enumerate(hist) if h[1] < a.convergence), None)
-    while n_conv is None and len(hist) < 200:
-        e, c = cc.iterate()
-        hist.append((e, c, cc.current_precision()))
-        if c < a.convergence:
-            n_conv = len(hist)
-    t_mixed_to_E = time.perf_counter() - tstart
-    e_mixed = hist[(n_conv or len(hist)) - 1][0]
+    sm = solve_summary(hm, a.convergence)
     dt2_mixed = max_dt2(cc)
     cc.close()
 
-    # ---- the other reduced arithmetic (FP32 MFMA when the mixed run was
-    # BF16x3): same policy, to convergence, for its energy and T2 error
+    # ---- the other reduced arithmetic (optional): energy and T2 error
     other = None
-    if a.compare_precisions and e64 is not None:
+    if a.compare_precisions and s64 is not None:
         prec2 = "fp32" if a.mixed == "bf16x3" else "bf16x3"
-        c2 = aq.RHFCCSD(ctx, ints, packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis, max_iterations=200,
-                        precision=prec2, refine_below=a.refine_below, min_reduced_flops=1e8)
-        t0 = time.perf_counter()
-        r2 = c2.run(callback=lambda h: log(f"{prec2} it {h['iter']:2d} E={h['energy']:.12f} conv={h['conv']:.2e} "
-                                             f"{h['seconds'] * 1e3:.0f} ms"))
-        t2s = time.perf_counter() - t0
-        h2 = c2.history
-        red = [h["seconds"] for h in h2 if h["precision"] != aq.AQ_PREC_FP64]
-        other = {"precision": prec2, "iterations": r2["iterations"], "converged": r2["converged"],
-                 "time_to_E_s": t2s, "iter_per_s": len(h2) / sum(h["seconds"] for h in h2),
-                 "reduced_iterations": len(red), "reduced_iter_per_s": len(red) / sum(red) if red else None,
-                 "dE_vs_fp64": r2["energy"] - e64, "max_abs_dT2_vs_fp64": max_dt2(c2)}
+        c2, h2 = timed_solve(aq, ctx, ints, a, prec2, prec2)
+        other = solve_summary(h2, a.convergence)
+        other.update({"precision": prec2, "dE_vs_fp64": other["energy"] - s64["energy"],
+                      "max_abs_dT2_vs_fp64": max_dt2(c2)})
         c2.close()
     if t2_64 is not None:
         t2_64.free()
 
-    value = steps / t_timed
-    reduced = [h for h in hist[a.warmup:a.warmup + steps] if h[2] != aq.AQ_PREC_FP64]
-    phase_rate = {["fp64", "fp32", "bf16x3"][p]: {"iterations": len(ts), "iter_per_s": len(ts) / sum(ts)}
-                  for p, ts in phase_t.items()}
     rf_red = roofline_of(timm, "gemm_bf16x3" if a.mixed == "bf16x3" else "gemm_f32", PEAK[a.mixed])
     rf_64 = roofline_of(timm, "gemm_f64", PEAK["fp64"])
-    # dominant kernel of the timed region
     cands = [r for r in (rf_red, rf_64) if r]
     dominant = max(cands, key=lambda r: r["launches"] * r["mean_ms"]) if cands else None
-    for r in (rf_red, rf_64, (out.get("fp64") or {}).get("roofline")):
+    for r in (rf_red, rf_64, (s64 or {}).get("roofline")):
         if r:
             r["traffic"] = TRAFFIC_PER_LAUNCH.get((r["kernel"], o, v))
             r["traffic_unit"] = "bytes per launch (rocprofv3 --pmc, copy-calibrated)" if r["traffic"] else None
 
+    steps = sm["iterations"]
+    t_iters = sm["time_to_E_s"]
     res = {
-        "metric": METRIC, "value": value, "unit": "iter/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
-        "ms_per_step": 1e3 * t_timed / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": f"{a.mixed}+fp64", "data": "synthetic",
+        "metric": METRIC, "value": steps / t_iters, "unit": "iter/s", "n_gpus": 1, "steps": steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 * t_iters / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": f"{a.mixed}+fp64", "data": "synthetic",
+        "timed_region": "the complete mixed solve: every iteration from the first to convergence "
+                        f"(conv < {a.convergence:g}), {a.mixed} phase and FP64 refinement, wall clock per iteration; "
+                        f"--steps {a.steps} does not shorten it (the metric is defined at the converged energy)",
+        "timed_window_wall_s": t_window,
         "config": {"workload": f"synthetic closed-shell RHF-CCSD o={o} v={v} nQ={nq} seed={a.seed}, full vvvv "
                                f"resident; {a.mixed} MFMA contractions until conv<{a.refine_below:g}, then FP64"
                                + ("; ladder on packed (anti)symmetric pairs" if a.packed_ladder else ""),
@@ -546,32 +597,47 @@ def main():
                    "flops_per_iter_executed": flops_per_iter_executed(o, v, a.packed_ladder),
                    "gemm_autotune": bool(a.autotune)},
         "roofline": dominant,
-        "mixed": {"iterations_to_converge": n_conv, "energy": e_mixed, "time_to_E_s": t_mixed_to_E,
-                  "reduced_iterations_in_timed_window": len(reduced), "phases": phase_rate,
-                  "roofline_reduced": rf_red,
-                  "roofline_fp64": rf_64},
-        "fp64": out.get("fp64"),
+        "mixed": dict(sm, roofline_reduced=rf_red, roofline_fp64=rf_64),
+        "reduced_phase_iter_per_s": sm["phases"].get(a.mixed, {}).get("iter_per_s"),
+        "fp64": s64,
     }
-    if e64 is not None:
-        res["dE_mixed_vs_fp64"] = e_mixed - e64
+    if s64 is not None:
+        res["dE_mixed_vs_fp64"] = sm["energy"] - s64["energy"]
         res["max_abs_dT2_mixed_vs_fp64"] = dt2_mixed
         res["other_reduced_precision"] = other
-        res["fp64_iter_per_s"] = out["fp64"]["iter_per_s"]
-        res["speedup_iter_per_s"] = value / out["fp64"]["iter_per_s"]
-        res["speedup_time_to_E"] = out["fp64"]["time_to_E_s"] / t_mixed_to_E
-    res["serving"] = "separate mode: python bench.py --serving (one JSON line of its own)"
+        res["fp64_iter_per_s"] = s64["iter_per_s"]
+        res["speedup_iter_per_s"] = sm["iter_per_s"] / s64["iter_per_s"]
+        res["speedup_time_to_E"] = s64["time_to_E_s"] / sm["time_to_E_s"]
+        res["reduced_phase_speedup_iter_per_s"] = (res["reduced_phase_iter_per_s"] or 0) / s64["iter_per_s"]
     tab = aq.autotune_table()
     res["gemm_autotune"] = {"buckets": len(tab),
-                            "variant_counts": {p: {str(v): sum(1 for e in tab if e["precision"] == c and e["variant"] == v)
-                                                   for v in (0, 1, 2, 3, 4)}
+                            "variant_counts": {p: {str(vv): sum(1 for e in tab if e["precision"] == c and e["variant"] == vv)
+                                                   for vv in (0, 1, 2, 3, 4)}
                                                for p, c in aq.PRECISIONS.items()}}
     ints.free()
+
+    # ---- the CPU leg's system on the GPU (FP64 solve to convergence)
+    cpu_sys = tuple(int(x) for x in a.cpu_sample.split(",")) if a.cpu_sample else None
+    gpu_small_rate = None
+    if cpu_sys:
+        cs, co, cv, _ = cpu_sys
+        si = aq.Integrals.synthetic(ctx, cs, co, cv, 3 * (co + cv))
+        c1, h1 = timed_solve(aq, ctx, si, a, "fp64", f"fp64 o={co} v={cv}")
+        s1 = solve_summary(h1, a.convergence)
+        gpu_small_rate = s1["iter_per_s"]
+        res["fp64_cpu_leg_system"] = {"o": co, "v": cv, "seed": cs, **s1}
+        c1.close()
+        si.free()
     ctx.close()
 
-    if a.cpu_sample:
-        so, sv, si = (int(x) for x in a.cpu_sample.split(","))
-        log(f"cpu baseline: {si} oracle iterations at o={so} v={sv}")
-        res["cpu_baseline"] = cpu_baseline(so, sv, si, a.seed, o, v)
+    # ---- serving stream (continuous batching), same run
+    if a.serving_in_line and a.jobs > 0:
+        res["serving"] = serve_stream(a, 4)
+
+    if cpu_sys:
+        cs, co, cv, ci = cpu_sys
+        log(f"cpu baseline: {ci} oracle iterations at o={co} v={cv}")
+        res["cpu_baseline"] = cpu_baseline(cs, co, cv, 3 * (co + cv), ci, gpu_small_rate, o, v)
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res))
