@@ -73,6 +73,7 @@ _SIGS = {
     "aq_tensor_lengths": [_P, _I64P],
     "aq_tensor_size": [_P, _I64P],
     "aq_tensor_data": [_P, C.POINTER(_P)],
+    "aq_tensor_touch": [_P],
     "aq_contract": [_P, C.c_double, _P, C.c_char_p, _P, C.c_char_p, C.c_double, _P, C.c_char_p],
     "aq_sum": [_P, C.c_double, _P, C.c_char_p, C.c_double, _P, C.c_char_p],
     "aq_scale": [_P, _P, C.c_double, C.c_char_p],
@@ -269,6 +270,16 @@ class Tensor:
             _ck(lib().aq_tensor_destroy(self.h))
         self.h = _P()
 
+    def data_ptr(self):
+        """Device address of the column-major data (aq_tensor_data)."""
+        p = _P()
+        _ck(lib().aq_tensor_data(self.h, C.byref(p)))
+        return p.value
+
+    def touch(self):
+        """Declare an outside write through data_ptr() (aq_tensor_touch)."""
+        _ck(lib().aq_tensor_touch(self.h))
+
     def write_all(self, arr):
         arr = np.asarray(arr, dtype=np.float64)
         if arr.shape != self.shape:
@@ -440,11 +451,46 @@ class RHFCCSD:
         return c.value, r.value, f.value
 
 
+# ---------------------------------------------------------------- DIIS
+
+_SIGS.update({
+    "aq_diis_create": [_P, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.POINTER(_P)],
+    "aq_diis_destroy": [_P],
+    "aq_diis_extrapolate": [_P, C.POINTER(_P), C.POINTER(_P)],
+    "aq_diis_coefficients": [_P, _DP, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+})
+
+
+class DIIS:
+    """Device DIIS extrapolator (aq_diis_*; DIIS<T>, diis.hpp:36-227)."""
+
+    def __init__(self, ctx, order, start=1, damping=0.0, nx=1, ndx=1):
+        self.ctx, self.order = ctx, order
+        h = _P()
+        _ck(lib().aq_diis_create(ctx.h, order, start, damping, nx, ndx, C.byref(h)))
+        self.h = h
+
+    def extrapolate(self, x, dx):
+        px = (_P * len(x))(*[t.h for t in x])
+        pdx = (_P * len(dx))(*[t.h for t in dx])
+        _ck(lib().aq_diis_extrapolate(self.h, px, pdx))
+
+    def coefficients(self):
+        buf = (C.c_double * (self.order + 1))()
+        n, retry = C.c_int(), C.c_int()
+        _ck(lib().aq_diis_coefficients(self.h, buf, self.order + 1, C.byref(n), C.byref(retry)))
+        return np.array(buf[:n.value]), bool(retry.value)
+
+    def close(self):
+        if self.h:
+            _ck(lib().aq_diis_destroy(self.h))
+        self.h = _P()
+
 # ---------------------------------------------------------------- serving
 
 class JobSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("o", C.c_int), ("v", C.c_int), ("nQ", C.c_int), ("sigma", C.c_double),
-                ("cfg", RHFCCSDConfig)]
+                ("cfg", RHFCCSDConfig), ("inputs", C.POINTER(_P)), ("fcidump", C.c_char_p)]
 
 
 class JobResult(C.Structure):
@@ -494,11 +540,21 @@ class Server:
         self.h = _P()
         _ck(lib().aq_server_create(device, int(hbm_budget), slots, float(max_bypass_s), C.byref(self.h)))
 
-    def submit(self, seed, o, v, nQ=0, sigma=0.0, config=None, **over):
+    def submit(self, seed, o, v, nQ=0, sigma=0.0, config=None, inputs=None, fcidump=None, **over):
+        """Queue a job: the synthetic system (seed, o, v, nQ, sigma), or the
+        caller's integrals (`inputs`: an Integrals on this device, borrowed
+        until the job is done), or an FCIDUMP file (`fcidump`: a path)."""
         spec = JobSpec(seed=seed, o=o, v=v, nQ=nQ, sigma=sigma,
                        cfg=config if config is not None else default_config(**over))
+        keep = None
+        if inputs is not None:
+            keep = (_P * 10)(*[inputs.t[k].h for k in FOCK + BLOCKS])
+            spec.inputs = C.cast(keep, C.POINTER(_P))
+        if fcidump is not None:
+            spec.fcidump = os.fsencode(str(fcidump))
         jid = C.c_int64()
         _ck(lib().aq_server_submit(self.h, C.byref(spec), C.byref(jid)))
+        del keep
         return jid.value
 
     def poll(self, jid):

@@ -4,6 +4,8 @@
 // here is global, so contexts on one device are independent.
 #include "internal.hpp"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cstring>
 
@@ -184,6 +186,11 @@ void resolve_timing(aq_ctx* ctx) {
 }
 
 }  // namespace aqamd
+
+uint64_t new_tensor_id() {
+    static std::atomic<uint64_t> next{1};
+    return next.fetch_add(1, std::memory_order_relaxed);
+}
 
 using namespace aqamd;
 

@@ -63,7 +63,6 @@ class DIIS {
     void extrapolate(const std::vector<Tensor*>& x, const std::vector<Tensor*>& dx) {
         const int n = nextrap_, n1 = n + 1;
         if (n <= 1) return;
-        AQ_CHECK(n <= 8, AQ_ERR_UNSUPPORTED, "DIIS order above 8 is not supported");
         // slot k holds iteration n-k (diis.hpp:82-85)
         std::rotate(old_x_.begin(), old_x_.end() - 1, old_x_.end());
         std::rotate(old_dx_.begin(), old_dx_.end() - 1, old_dx_.end());
@@ -81,12 +80,19 @@ class DIIS {
         int nreal = 1;
         while (nreal < n && !old_dx_[nreal].empty()) ++nreal;
         std::vector<double> p(nreal, 0.0);
+        // up to kDotSlots history slots per launch (the context's reduction
+        // slots); each inner product is reduced on its own, so the grouping
+        // does not change any value
+        constexpr int kDotSlots = 8;
         for (int j = 0; j < ndx_; ++j) {
-            std::vector<const double*> ys;
-            for (int i = 0; i < nreal; ++i) ys.push_back(dptr(old_dx_[i][j]));
-            launch_multi_dot(ctx_, dptr(*dx[j]), ys.data(), nreal, tsize(*dx[j]), 0);
-            AQ_HIP(hipStreamSynchronize(ctx_->stream));
-            for (int i = 0; i < nreal; ++i) p[i] += ctx_->h_result[i];
+            for (int first = 0; first < nreal; first += kDotSlots) {
+                const int kk = std::min(kDotSlots, nreal - first);
+                std::vector<const double*> ys;
+                for (int i = 0; i < kk; ++i) ys.push_back(dptr(old_dx_[first + i][j]));
+                launch_multi_dot(ctx_, dptr(*dx[j]), ys.data(), kk, tsize(*dx[j]), 0);
+                AQ_HIP(hipStreamSynchronize(ctx_->stream));
+                for (int i = 0; i < kk; ++i) p[first + i] += ctx_->h_result[i];
+            }
         }
         resolve_timing(ctx_);
         E(0, 0) = p[0];

@@ -823,29 +823,36 @@ void launch_multi_dot(aq_ctx* ctx, const double* x, const double* const* ys, int
 }
 
 // out = sum_k c_k * xs_k, accumulated in the reference's order
-// (out = x0*c0; out += x_k*c_k, diis.hpp:205-226)
+// (out = x0*c0; out += x_k*c_k, diis.hpp:205-226).  More than MAXK terms run
+// as several launches; each later launch continues from `out` (cont = 1), so
+// every element is summed left to right exactly as in one pass.
 struct CoefList {
     double c[MAXK];
 };
 
-__global__ __launch_bounds__(TPB) void k_lincomb(double* out, PtrList xs, CoefList c, int nk, int64_t n) {
+__global__ __launch_bounds__(TPB) void k_lincomb(double* out, PtrList xs, CoefList c, int nk, int64_t n, int cont) {
     for (int64_t i = int64_t(blockIdx.x) * TPB + threadIdx.x; i < n; i += int64_t(gridDim.x) * TPB) {
-        double s = xs.p[0][i] * c.c[0];
+        double s = cont ? out[i] + xs.p[0][i] * c.c[0] : xs.p[0][i] * c.c[0];
         for (int k = 1; k < nk; ++k) s = s + xs.p[k][i] * c.c[k];
         out[i] = s;
     }
 }
 
 void launch_lincomb(aq_ctx* ctx, double* out, const double* const* xs, const double* c, int k, int64_t n) {
-    AQ_CHECK(k >= 1 && k <= MAXK, AQ_ERR_INVALID, "lincomb arity");
-    PtrList pl{};
-    CoefList cl{};
-    for (int i = 0; i < k; ++i) {
-        pl.p[i] = xs[i];
-        cl.c[i] = c[i];
+    AQ_CHECK(k >= 1, AQ_ERR_INVALID, "lincomb arity");
+    for (int first = 0; first < k; first += MAXK) {
+        const int kk = std::min(MAXK, k - first);
+        PtrList pl{};
+        CoefList cl{};
+        for (int i = 0; i < kk; ++i) {
+            pl.p[i] = xs[first + i];
+            cl.c[i] = c[first + i];
+        }
+        TimedLaunch tl(ctx, "diis_lincomb", 2.0 * n * kk, 8.0 * n * (kk + 1 + (first > 0)));
+        hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n, TPB * 4)), dim3(TPB), 0, ctx->stream, out, pl, cl, kk, n,
+                           int(first > 0));
+        AQ_HIP(hipGetLastError());
     }
-    TimedLaunch tl(ctx, "diis_lincomb", 2.0 * n * k, 8.0 * n * (k + 1));
-    hipLaunchKernelGGL(k_lincomb, dim3(grid_for(n, TPB * 4)), dim3(TPB), 0, ctx->stream, out, pl, cl, k, n);
     AQ_HIP(hipGetLastError());
 }
 

@@ -31,6 +31,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -371,12 +372,13 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
 
 // ------------------------------------------------------------------ BF16x3
 
-// PF = 1: the next k-tile's global loads are issued before this tile's
-// MFMAs.  PF = 2: two register sets, the tile two steps ahead is loaded, so a
-// load has two tiles of MFMA work (~1500 cycles) to arrive instead of one.
-template <int BM, int BN, int WM, int WN, int PF, bool REMAP>
+// NT = 64*WM*WN threads (4 waves: one per SIMD; 8 waves: two per SIMD, each
+// holding half the accumulators, so one wave's LDS and barrier waits are
+// covered by the other's MFMAs).
+template <int BM, int BN, int WM, int WN, bool REMAP>
 __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, int bz) {
     constexpr int BK = 32;
+    constexpr int NT = 64 * WM * WN;
     // LDS rows of BK bf16 (64 bytes, four 16-byte chunks), chunk c of row r
     // stored at chunk c ^ ((r >> 1) & 3).  Measured on gfx950 (bank-conflict
     // counter): this layout serves both the MFMA operand reads (lane -> row
@@ -388,9 +390,8 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     constexpr int WTM = BM / WM, WTN = BN / WN;
     constexpr int FM = WTM / 16, FN = WTN / 16;
     constexpr int VA = BM * BK / 8, VB = BN * BK / 8;   // 8-element vectors per plane
-    constexpr int NA = (VA + 255) / 256, NB = (VB + 255) / 256;
-    static_assert(WM * WN == 4, "4 waves");
-    static_assert(PF == 1 || PF == 2, "prefetch depth");
+    constexpr int NA = (VA + NT - 1) / NT, NB = (VB + NT - 1) / NT;
+    static_assert(WM * WN == 4 || WM * WN == 8, "4 or 8 waves");
 
     __shared__ i16x8 dummy_align;                   // keeps LDS 16-byte aligned
     __shared__ __attribute__((aligned(16))) unsigned short As[2][2][BM][KP];   // [buf][hi/lo][m][k]
@@ -420,7 +421,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
         const i16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
-            const int e = tid + 256 * r;
+            const int e = tid + NT * r;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
             const int64_t m = m0 + mm, k = k0 + kk;
             i16x8 h = z, l = z;
@@ -433,7 +434,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
-            const int e = tid + 256 * r;
+            const int e = tid + NT * r;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
             const int64_t n = n0 + nn, k = k0 + kk;
             i16x8 h = z, l = z;
@@ -448,7 +449,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     auto store = [&](int buf, const i16x8 (&xa)[2][NA], const i16x8 (&xb)[2][NB]) {
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
-            const int e = tid + 256 * r;
+            const int e = tid + NT * r;
             if (e >= VA) continue;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
             *reinterpret_cast<i16x8*>(&As[buf][0][mm][swz(mm, kk)]) = xa[0][r];
@@ -456,7 +457,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
-            const int e = tid + 256 * r;
+            const int e = tid + NT * r;
             if (e >= VB) continue;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
             *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][swz(nn, kk)]) = xb[0][r];
@@ -470,69 +471,58 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // src0 (16 x 32): lane holds row lane%16, k = 8*(lane/16) .. +7  (from the B tile)
-    // src1 (32 x 16): lane holds col lane%16, k = 8*(lane/16) .. +7  (from the A tile)
-    auto compute = [&](int buf) {
-        const int kq = 8 * (lane / 16);
-        bf16x8 xh[FN], xl[FN], yh[FM], yl[FM];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
-            const int nn = wn * WTN + j * 16 + lane % 16;
-            xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][swz(nn, kq)]));
-            xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][swz(nn, kq)]));
-        }
-        auto load_a = [&](int i) {
-            const int mm = wm * WTM + i * 16 + lane % 16;
-            yh[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][swz(mm, kq)]));
-            yl[i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][swz(mm, kq)]));
-        };
-        // a 128 x 128 wave tile holds its accumulators in all 256 AGPRs:
-        // read each A fragment pair just before its row of blocks
-        constexpr bool lazy_a = FM * FN >= 64;
-        if (!lazy_a) {
-#pragma unroll
-            for (int i = 0; i < FM; ++i) load_a(i);
-        }
-#pragma unroll
-        for (int i = 0; i < FM; ++i) {
-            if (lazy_a) load_a(i);
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i], acc[i][j], 0, 0, 0);
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i], acc[i][j], 0, 0, 0);
-            }
-        }
-    };
-
+    // MFMA operands: src0 (16 x 32) from the B tile, lane holds row lane%16,
+    // k = 8*(lane/16) .. +7; src1 (32 x 16) from the A tile, lane holds column
+    // lane%16, the same k.
     const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
     i16x8 ra0[2][NA], rb0[2][NB];
-    if (PF == 1) {
-        if (nk > 0) load(kbeg, ra0, rb0);
+    {
+        // One barrier per k-tile.  Tile kt+1 (in registers) is stored to the
+        // other LDS buffer halfway through tile kt's MFMAs, and the loads of
+        // tile kt+2 are issued right after that store, so a global load has a
+        // whole k-tile of MFMA work to arrive.  Inside a tile the A fragments
+        // of row i+1 are read while row i's MFMAs run (two register pairs),
+        // so the MFMAs never wait for an LDS read they need next.
+        // The MFMA sequence of every accumulator (k-tiles in order, three
+        // products per block in the order lo*hi, hi*lo, hi*hi) is the one
+        // every other BF16x3 path uses.
+        if (nk > 0) {
+            load(kbeg, ra0, rb0);
+            store(0, ra0, rb0);
+            if (nk > 1) load(kbeg + BK, ra0, rb0);
+            __syncthreads();
+        }
+        const int kq = 8 * (lane / 16);
         for (int64_t kt = 0; kt < nk; ++kt) {
             const int buf = int(kt & 1);
-            store(buf, ra0, rb0);
+            bf16x8 xh[FN], xl[FN], yh[2], yl[2];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                const int nn = wn * WTN + j * 16 + lane % 16;
+                xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][swz(nn, kq)]));
+                xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][swz(nn, kq)]));
+            }
+            auto read_a = [&](int i, int slot) {
+                const int mm = wm * WTM + i * 16 + lane % 16;
+                yh[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][swz(mm, kq)]));
+                yl[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][swz(mm, kq)]));
+            };
+            read_a(0, 0);
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                if (i + 1 < FM) read_a(i + 1, (i + 1) & 1);
+                if (i == FM / 2 && kt + 1 < nk) {
+                    store(buf ^ 1, ra0, rb0);
+                    if (kt + 2 < nk) load(kbeg + (kt + 2) * BK, ra0, rb0);
+                }
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i & 1], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i & 1], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i & 1], acc[i][j], 0, 0, 0);
+                }
+            }
             __syncthreads();
-            if (kt + 1 < nk) load(kbeg + (kt + 1) * BK, ra0, rb0);
-            __builtin_amdgcn_sched_barrier(0);
-            compute(buf);
-        }
-    } else {
-        i16x8 ra1[2][NA], rb1[2][NB];
-        if (nk > 0) load(kbeg, ra0, rb0);
-        if (nk > 1) load(kbeg + BK, ra1, rb1);
-        for (int64_t kt = 0; kt < nk; kt += 2) {
-            store(0, ra0, rb0);
-            __syncthreads();
-            if (kt + 2 < nk) load(kbeg + (kt + 2) * BK, ra0, rb0);
-            __builtin_amdgcn_sched_barrier(0);
-            compute(0);
-            if (kt + 1 >= nk) break;
-            store(1, ra1, rb1);
-            __syncthreads();
-            if (kt + 3 < nk) load(kbeg + (kt + 3) * BK, ra1, rb1);
-            __builtin_amdgcn_sched_barrier(0);
-            compute(1);
         }
     }
 
@@ -556,9 +546,9 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     }
 }
 
-template <int BM, int BN, int WM, int WN, int PF>
-__global__ __launch_bounds__(256, PF == 2 ? 2 : 1) void k_gemm_bf16x3(GemmArgs g) {
-    gemm_tile_bf16x3<BM, BN, WM, WN, PF, true>(g, blockIdx.x, blockIdx.z);
+template <int BM, int BN, int WM, int WN>
+__global__ __launch_bounds__(64 * WM * WN, 1) void k_gemm_bf16x3(GemmArgs g) {
+    gemm_tile_bf16x3<BM, BN, WM, WN, true>(g, blockIdx.x, blockIdx.z);
 }
 
 // ---------------------------------------------------------- grouped launch
@@ -582,22 +572,31 @@ __device__ __forceinline__ int group_problem(const GemmGroup& G, int64_t bid) {
     return p;
 }
 
+// Persistent: the grid holds at most as many blocks as the device keeps
+// resident, and each block walks the group's tiles with a stride of the grid
+// size, so one launch of a fixed grid covers every problem of the group.
 template <typename T, int BM, int BN, int WM, int WN, bool TA, bool TB, int BK, int MINW>
 __global__ __launch_bounds__(256, MINW) void k_gemm_grouped(GemmGroup G) {
-    const int64_t bid = blockIdx.x;
-    const int p = group_problem(G, bid);
-    const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
-    const int64_t local = bid - G.first[p];
-    gemm_tile<T, BM, BN, WM, WN, TA, TB, BK, false>(G.p[p], local % tiles, int(local / tiles));
+    const int64_t total = G.first[G.n];
+    for (int64_t bid = blockIdx.x; bid < total; bid += gridDim.x) {
+        const int p = group_problem(G, bid);
+        const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
+        const int64_t local = bid - G.first[p];
+        gemm_tile<T, BM, BN, WM, WN, TA, TB, BK, false>(G.p[p], local % tiles, int(local / tiles));
+        __syncthreads();   // the next tile reuses the LDS buffers
+    }
 }
 
 template <int BM, int BN, int WM, int WN>
 __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_grouped(GemmGroup G) {
-    const int64_t bid = blockIdx.x;
-    const int p = group_problem(G, bid);
-    const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
-    const int64_t local = bid - G.first[p];
-    gemm_tile_bf16x3<BM, BN, WM, WN, 1, false>(G.p[p], local % tiles, int(local / tiles));
+    const int64_t total = G.first[G.n];
+    for (int64_t bid = blockIdx.x; bid < total; bid += gridDim.x) {
+        const int p = group_problem(G, bid);
+        const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
+        const int64_t local = bid - G.first[p];
+        gemm_tile_bf16x3<BM, BN, WM, WN, false>(G.p[p], local % tiles, int(local / tiles));
+        __syncthreads();   // the next tile reuses the LDS buffers
+    }
 }
 
 // ---------------------------------------------------------------- dispatch
@@ -631,9 +630,9 @@ static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid, int variant) {
 // single wave per SIMD exposes (FP64 35 -> 56 TF/s; FP32 unchanged at ~108).
 int default_gemm_variant(Fmt fmt) { return fmt == FMT_BF16X2 ? 0 : 2; }
 
-template <int BM, int BN, int WM, int WN, int PF = 1>
+template <int BM, int BN, int WM, int WN>
 static void run_cfg_bf16(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
-    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN, PF>), grid, dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0, ctx->stream, a);
 }
 
 // Kernel launch for one (format, block shape, variant); the caller checked the shape.
@@ -649,8 +648,8 @@ static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const Ge
         else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
         else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
     } else {
-        if (BM == 256 && BN == 256) run_cfg_bf16<256, 256, 2, 2>(ctx, a, grid);
-        else if (BM == 128 && BN == 128 && variant == 3) run_cfg_bf16<128, 128, 2, 2, 2>(ctx, a, grid);
+        if (BM == 256 && BN == 256 && variant == 3) run_cfg_bf16<256, 256, 2, 4>(ctx, a, grid);
+        else if (BM == 256 && BN == 256) run_cfg_bf16<256, 256, 2, 2>(ctx, a, grid);
         else if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
         else if (BM == 256) run_cfg_bf16<256, 128, 2, 2>(ctx, a, grid);
         else if (BN == 256) run_cfg_bf16<128, 256, 2, 2>(ctx, a, grid);
@@ -662,14 +661,15 @@ static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const Ge
 
 // Block shape of a variant: BF16x3 variants 1 and 2 enlarge the 128 x 128
 // block to 256 x 128 / 128 x 256 (128 x 64 / 64 x 128 per wave), which reads a
-// third less LDS per MFMA, and variant 4 to 256 x 256 (128 x 128 per wave, the
-// accumulators in all 256 AGPRs, one wave per SIMD); the FP64/FP32 variants
-// change BK and occupancy only.
+// third less LDS per MFMA, variant 4 to 256 x 256 with four waves (128 x 128
+// per wave, the accumulators in all 256 AGPRs, one wave per SIMD) and variant
+// 3 to 256 x 256 with eight waves (128 x 64 per wave, two waves per SIMD); the
+// FP64/FP32 variants change BK and occupancy only.
 static void variant_shape(Fmt fmt, int variant, int64_t M, int64_t N, int& BM, int& BN) {
     if (fmt == FMT_BF16X2 && BM == 128 && BN == 128) {
         if (variant == 1 && M >= 256) BM = 256;
         else if (variant == 2 && N >= 256) BN = 256;
-        else if (variant == 4 && M >= 256 && N >= 256) { BM = 256; BN = 256; }
+        else if ((variant == 3 || variant == 4) && M >= 256 && N >= 256) { BM = 256; BN = 256; }
     }
 }
 
@@ -736,7 +736,7 @@ static int tuned_variant(aq_ctx* ctx, const GemmDesc& d, int BM, int BN, const G
         for (int v = 0; v < nvar; ++v) {
             int bm = BM, bn = BN;
             variant_shape(d.fmt, v, d.M, d.N, bm, bn);
-            if (d.fmt == FMT_BF16X2 && (v == 1 || v == 2 || v == 4) && bm == BM && bn == BN) continue;   // = variant 0
+            if (d.fmt == FMT_BF16X2 && v >= 1 && bm == BM && bn == BN) continue;   // = variant 0
             const int64_t tiles = ((d.M + bm - 1) / bm) * ((d.N + bn - 1) / bn);
             dim3 grid(unsigned(tiles), 1, unsigned(a.batch * a.nsplit));
             dispatch(ctx, d.fmt, bm, bn, v, t, grid);           // warm
@@ -778,8 +778,12 @@ bool group_shape(Fmt fmt, int BM, int BN) {
     return (fmt == FMT_F64 || fmt == FMT_BF16X2) && BM == BN && (BM == 128 || BM == 64);
 }
 
-void launch_grouped(const GroupKey& k, const GemmGroup& G, int64_t blocks, hipStream_t s) {
-    const dim3 grid{unsigned(blocks)}, block{256};
+// Resident blocks of the grouped kernels: 2 per CU (FP64 128 x 128 runs two
+// waves per SIMD; the BF16x3 128 x 128 and the 64 x 64 tiles fit two blocks'
+// LDS per CU), times the CU count.
+void launch_grouped(const GroupKey& k, const GemmGroup& G, int64_t blocks, hipStream_t s, int cus) {
+    const int64_t resident = int64_t(std::max(cus, 1)) * 2;
+    const dim3 grid{unsigned(std::min(blocks, resident))}, block{256};
 #define AQ_GROUPED(BM_, BK_, MW_)                                                                                      \
     do {                                                                                                           \
         if (k.ta && k.tb)                                                                                          \
@@ -806,6 +810,7 @@ void launch_grouped(const GroupKey& k, const GemmGroup& G, int64_t blocks, hipSt
 
 struct GemmBatcher {
     int device = 0;
+    int cus = 0;                          // compute units of the device
     hipStream_t stream = nullptr;
     std::chrono::microseconds window{0};
     std::mutex mu;
@@ -825,9 +830,28 @@ struct GemmBatcher {
     };
     std::vector<Item*> pending;           // in submission order
     std::vector<hipEvent_t> spare;
+    // pacing: completion events of grouped launches that may still run
+    // (oldest first).  While kPaceDepth of them run, a new GEMM waits for
+    // same-key partners (up to the window) instead of launching alone: the
+    // device has queued work, so the wait costs no device time and lets the
+    // other slots' GEMMs join the next launch.
+    std::deque<hipEvent_t> inflight;
+    std::vector<hipEvent_t> pace_spare;
     int64_t launches = 0, problems = 0;
     int max_group = 0;
 };
+
+constexpr int kPaceDepth = 2;
+
+// Grouped launches still running (mu held); retires the finished ones.
+static int batcher_busy(GemmBatcher* b) {
+    while (!b->inflight.empty() && hipEventQuery(b->inflight.front()) == hipSuccess) {
+        b->pace_spare.push_back(b->inflight.front());
+        b->inflight.pop_front();
+    }
+    (void)hipGetLastError();
+    return int(b->inflight.size());
+}
 
 // Launch up to kMaxGroup pending items of `key` as one grouped kernel (mu held).
 static void launch_group(GemmBatcher* b, const GroupKey& key) {
@@ -860,9 +884,18 @@ static void launch_group(GemmBatcher* b, const GroupKey& key) {
         } else {
             AQ_HIP(hipEventCreateWithFlags(&L->done, hipEventDisableTiming));
         }
-        launch_grouped(key, G, blocks, b->stream);
+        launch_grouped(key, G, blocks, b->stream, b->cus);
         AQ_HIP(hipGetLastError());
         AQ_HIP(hipEventRecord(L->done, b->stream));
+        hipEvent_t pace = nullptr;
+        if (!b->pace_spare.empty()) {
+            pace = b->pace_spare.back();
+            b->pace_spare.pop_back();
+        } else {
+            AQ_HIP(hipEventCreateWithFlags(&pace, hipEventDisableTiming));
+        }
+        AQ_HIP(hipEventRecord(pace, b->stream));
+        b->inflight.push_back(pace);
     } catch (...) {
         if (L->done) b->spare.push_back(L->done);
         delete L;
@@ -900,11 +933,13 @@ static void batcher_submit(aq_ctx* ctx, const GroupKey& key, const GemmArgs& a, 
         int same = 0;
         for (const auto* x : b->pending) same += x->key == key ? 1 : 0;
         const int want = std::min(std::max(b->in_stage, 1), kMaxGroup);
-        if (same >= want || std::chrono::steady_clock::now() >= deadline) {
+        const auto now = std::chrono::steady_clock::now();
+        if (same >= want || now >= deadline || batcher_busy(b) < kPaceDepth) {
             launch_group(b, key);
             continue;
         }
-        b->cv.wait_until(lk, deadline);
+        // poll the device at a short interval (no completion callback)
+        b->cv.wait_until(lk, std::min(deadline, now + std::chrono::microseconds(20)));
     }
     AQ_CHECK(!item.failed, AQ_ERR_HIP, "grouped GEMM launch failed");
     GemmBatcher::Launch* L = item.launch;
@@ -922,6 +957,7 @@ GemmBatcher* batcher_create(int device, int window_us) {
     b->device = device;
     b->window = std::chrono::microseconds(window_us);
     AQ_HIP(hipSetDevice(device));
+    AQ_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, device));
     AQ_HIP(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
     return b.release();
 }
@@ -931,6 +967,8 @@ void batcher_destroy(GemmBatcher* b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     for (auto e : b->spare) (void)hipEventDestroy(e);
+    for (auto e : b->inflight) (void)hipEventDestroy(e);
+    for (auto e : b->pace_spare) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(b->stream);
     delete b;
 }

@@ -98,6 +98,11 @@ struct CaptureLog {
 
 }  // namespace aqamd
 
+// Tensor ids are unique in the process (not per context): a context's operand
+// cache keys copies by (id, version), and a context may read tensors that
+// another context created (a served job's borrowed inputs).
+uint64_t new_tensor_id();
+
 struct aq_tensor {
     aq_ctx* ctx = nullptr;
     int ndim = 0;
@@ -122,7 +127,6 @@ struct aq_ctx {
     double min_reduced_flops = 0.0;
     int gemm_variant[3] = {-1, -1, -1};   // per format; -1 = library default
     bool autotune = false;                // pick GEMM variants per shape bucket by timing
-    uint64_t next_id = 1;
 
     // scratch pool: exact-size free lists
     std::map<size_t, std::vector<void*>> pool_free;

@@ -560,7 +560,7 @@ static double dot_impl(aq_ctx* ctx, const aq_tensor* tA, const char* ia, const a
     s.size = 1;
     Scratch buf(ctx, 256);
     s.data = buf.as<double>();
-    s.id = ctx->next_id++;
+    s.id = new_tensor_id();
     contract_impl(ctx, 1.0, tA, ia, tB, ib, 0.0, &s, "");
     double v = 0.0;
     AQ_HIP(hipMemcpyAsync(&v, s.data, 8, hipMemcpyDeviceToHost, ctx->stream));
@@ -597,7 +597,7 @@ int aq_tensor_create(aq_ctx* ctx, int ndim, const int64_t* len, int zero, aq_ten
             delete t;
             throw;
         }
-        t->id = ctx->next_id++;
+        t->id = new_tensor_id();
         if (zero) AQ_HIP(hipMemsetAsync(t->data, 0, size_t(std::max<int64_t>(t->size, 1)) * 8, ctx->stream));
         *out = t;
     });
@@ -660,6 +660,13 @@ int aq_tensor_data(const aq_tensor* t, double** dptr) {
     return guarded([&] {
         AQ_CHECK(t && dptr, AQ_ERR_INVALID, "null argument");
         *dptr = t->data;
+    });
+}
+
+int aq_tensor_touch(aq_tensor* t) {
+    return guarded([&] {
+        AQ_CHECK(t, AQ_ERR_INVALID, "null argument");
+        bump(t);
     });
 }
 

@@ -61,6 +61,98 @@ struct aq_rhfccsd {
         return cfg.precision;
     }
 
+    // Versions of the borrowed inputs that setup derived quantities from:
+    // the denominators (fAB, fIJ), the spin-adapted blocks (VABIJ, VAIJB,
+    // VAIBJ) and the packed ladder operators (VABCD, VABIJ).  A caller that
+    // rewrites an input and calls aq_tensor_touch bumps its version; the next
+    // iteration (or Hbar) rebuilds what was derived from it.  Inputs that the
+    // contractions read directly need nothing more: cached operand copies are
+    // keyed by the version.
+    uint64_t v_fAB = 0, v_fIJ = 0, v_VABIJ = 0, v_VAIJB = 0, v_VAIBJ = 0, v_VABCD = 0;
+    static uint64_t ver(const Tensor& t) { return t.handle()->version; }
+
+    void note_input_versions() {
+        v_fAB = ver(fAB);
+        v_fIJ = ver(fIJ);
+        v_VABIJ = ver(VABIJ);
+        v_VAIJB = ver(VAIJB);
+        v_VAIBJ = ver(VAIBJ);
+        v_VABCD = ver(VABCD);
+    }
+
+    void refresh_inputs() {
+        const bool f = ver(fAB) != v_fAB || ver(fIJ) != v_fIJ;
+        const bool sa = ver(VABIJ) != v_VABIJ || ver(VAIJB) != v_VAIJB || ver(VAIBJ) != v_VAIBJ;
+        const bool pk = ver(VABCD) != v_VABCD || ver(VABIJ) != v_VABIJ;
+        if (!f && !sa && !pk) return;
+        tensor::check(aq_ctx_set_precision(ctx, AQ_PREC_FP64, 0.0));
+        if (f) derive_denominators();
+        if (sa) derive_sa_blocks();
+        if (pk) derive_packed();
+        note_input_versions();
+    }
+
+    // Denominator: dA = -f_aa, dI = +f_ii (denominator.hpp:48-67)
+    void derive_denominators() {
+        std::vector<int64_t> ka(v), ki(o);
+        for (int a = 0; a < v; ++a) ka[a] = int64_t(a) + int64_t(a) * v;
+        for (int i = 0; i < o; ++i) ki[i] = int64_t(i) + int64_t(i) * o;
+        dA.assign(v, 0.0);
+        dI.assign(o, 0.0);
+        tensor::check(aq_read(ctx, fAB.handle(), v, ka.data(), dA.data()));
+        tensor::check(aq_read(ctx, fIJ.handle(), o, ki.data(), dI.data()));
+        for (auto& x : dA) x = -x;
+        if (!d_dA.handle()) {
+            d_dA = Tensor(ctx, std::vector<int64_t>{v}, true);
+            d_dI = Tensor(ctx, std::vector<int64_t>{o}, true);
+        }
+        d_dA.setAllData(dA);
+        d_dI.setAllData(dI);
+    }
+
+    void derive_sa_blocks() {
+        VABIJSA["abij"] = 2 * VABIJ["abij"];           // :88
+        VABIJSA["abij"] -= VABIJ["abji"];              // :89
+
+        VAIJBSA["aijb"] = 2 * VAIJB["aijb"];           // :91
+        VAIJBSA["aijb"] -= VAIBJ["aibj"];              // :92
+    }
+
+    // The ladder can run on the (anti)symmetric packed forms when
+    // <ab|ef> = <ba|fe> and <ab|ij> = <ba|ji> (so Tau(ef,ij) = Tau(fe,ji)).
+    void derive_packed() {
+        packed = false;
+        if (cfg.packed_ladder && o >= 2 && v >= 2) {
+            const double dV = pair_asymmetry(ctx, dptr(VABCD), v, v, 0);
+            const double dI = pair_asymmetry(ctx, dptr(VABIJ), v, o, 0);
+            const double nV = VABCD.norm(0), nI = VABIJ.norm(0);
+            packed = dV <= 1e-13 * (1.0 + nV) && dI <= 1e-13 * (1.0 + nI);
+        }
+        if (!packed) {
+            Vp = Tensor();
+            Vm = Tensor();
+            Tp = Tensor();
+            Tm = Tensor();
+            Sp = Tensor();
+            Sm = Tensor();
+            return;
+        }
+        const int64_t P = int64_t(v) * (v + 1) / 2, Q = int64_t(v) * (v - 1) / 2;
+        const int64_t Op = int64_t(o) * (o + 1) / 2, Om = int64_t(o) * (o - 1) / 2;
+        auto mk = [&](std::initializer_list<int64_t> l) { return Tensor(ctx, std::vector<int64_t>(l), true); };
+        if (!Vp.handle()) {
+            Vp = mk({P, P});
+            Vm = mk({Q, Q});
+            Tp = mk({P, Op});
+            Tm = mk({Q, Om});
+            Sp = mk({P, Op});
+            Sm = mk({Q, Om});
+        }
+        launch_ladder_pack_v(ctx, dptr(VABCD), v, dptr(Vp), dptr(Vm));
+        bump_t(Vp);
+        bump_t(Vm);
+    }
+
     // rhfccsd.cxx:45-98
     void setup() {
         const int64_t vo[2] = {v, o}, vvoo[4] = {v, v, o, o}, voov[4] = {v, o, o, v};
@@ -86,21 +178,7 @@ struct aq_rhfccsd {
         WAMEI = mk({v, o, v, o});
         WAMIE = mk({v, o, o, v});
 
-        // Denominator: dA = -f_aa, dI = +f_ii (denominator.hpp:48-67)
-        {
-            std::vector<int64_t> ka(v), ki(o);
-            for (int a = 0; a < v; ++a) ka[a] = int64_t(a) + int64_t(a) * v;
-            for (int i = 0; i < o; ++i) ki[i] = int64_t(i) + int64_t(i) * o;
-            dA.assign(v, 0.0);
-            dI.assign(o, 0.0);
-            tensor::check(aq_read(ctx, fAB.handle(), v, ka.data(), dA.data()));
-            tensor::check(aq_read(ctx, fIJ.handle(), o, ki.data(), dI.data()));
-            for (auto& x : dA) x = -x;
-            d_dA = mk({v});
-            d_dI = mk({o});
-            d_dA.setAllData(dA);
-            d_dI.setAllData(dI);
-        }
+        derive_denominators();
 
         T1 = fAI;                                      // :79
         T1.weight({&dA, &dI});                         // :80
@@ -110,35 +188,12 @@ struct aq_rhfccsd {
         Tau["abij"] = T2["abij"];                      // :85
         Tau["abij"] += T1["ai"] * T1["bj"];            // :86
 
-        VABIJSA["abij"] = 2 * VABIJ["abij"];           // :88
-        VABIJSA["abij"] -= VABIJ["abji"];              // :89
-
-        VAIJBSA["aijb"] = 2 * VAIJB["aijb"];           // :91
-        VAIJBSA["aijb"] -= VAIBJ["aibj"];              // :92
+        derive_sa_blocks();                            // :88-92
 
         mp2 = 2 * scalar(fAI["ai"] * T1["ai"]) + scalar(VABIJSA["abij"] * Tau["abij"]);   // :94
 
-        // The ladder can run on the (anti)symmetric packed forms when
-        // <ab|ef> = <ba|fe> and <ab|ij> = <ba|ji> (so Tau(ef,ij) = Tau(fe,ji)).
-        if (cfg.packed_ladder && o >= 2 && v >= 2) {
-            const double dV = pair_asymmetry(ctx, dptr(VABCD), v, v, 0);
-            const double dI = pair_asymmetry(ctx, dptr(VABIJ), v, o, 0);
-            const double nV = VABCD.norm(0), nI = VABIJ.norm(0);
-            packed = dV <= 1e-13 * (1.0 + nV) && dI <= 1e-13 * (1.0 + nI);
-        }
-        if (packed) {
-            const int64_t P = int64_t(v) * (v + 1) / 2, Q = int64_t(v) * (v - 1) / 2;
-            const int64_t Op = int64_t(o) * (o + 1) / 2, Om = int64_t(o) * (o - 1) / 2;
-            Vp = mk({P, P});
-            Vm = mk({Q, Q});
-            Tp = mk({P, Op});
-            Tm = mk({Q, Om});
-            Sp = mk({P, Op});
-            Sm = mk({Q, Om});
-            launch_ladder_pack_v(ctx, dptr(VABCD), v, dptr(Vp), dptr(Vm));
-            bump_t(Vp);
-            bump_t(Vm);
-        }
+        derive_packed();
+        note_input_versions();
         diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));   // :98
     }
 
@@ -319,6 +374,7 @@ struct aq_rhfccsd {
 
     // rhfccsd.cxx:194-327
     void iterate() {
+        refresh_inputs();
         const int prec = next_precision();
         if (cfg.diis_reset_on_refine && iter > 0 && prec == AQ_PREC_FP64 && last_prec != AQ_PREC_FP64)
             diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));
@@ -354,6 +410,7 @@ struct aq_rhfccsd {
 
     // Hbar products of the current amplitudes (rhfccsd.cxx:108-188), in FP64.
     void hbar(bool with_wabej) {
+        refresh_inputs();
         tensor::check(aq_ctx_set_precision(ctx, AQ_PREC_FP64, 0.0));
         Tau["abij"] = T2["abij"];                                  // :119
         Tau["abij"] += T1["ai"] * T1["bj"];                        // :120
@@ -468,7 +525,7 @@ int aq_rhfccsd_create(aq_ctx* ctx, const aq_tensor* const* f3, const aq_tensor* 
         cc->ctx = ctx;
         if (cfg) cc->cfg = *cfg;
         else aq_rhfccsd_default_config(&cc->cfg);
-        AQ_CHECK(cc->cfg.diis_order >= 0 && cc->cfg.diis_order <= 8, AQ_ERR_UNSUPPORTED, "DIIS order must be 0..8");
+        AQ_CHECK(cc->cfg.diis_order >= 0, AQ_ERR_INVALID, "DIIS order must be >= 0");
         cc->o = int(o);
         cc->v = int(v);
         auto b = [&](const aq_tensor* t) { return Tensor::borrow(ctx, const_cast<aq_tensor*>(t)); };

@@ -20,11 +20,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <mutex>
+#include <string>
 #include <thread>
 
 namespace aqamd {
@@ -33,11 +35,22 @@ struct Job {
     aq_job_spec spec{};
     aq_job_result res{};
     double t_queued = 0.0;
+    // caller-supplied integrals (spec.inputs / spec.fcidump point here after submit)
+    std::vector<aq_tensor*> inputs;
+    std::string fcidump;
 };
+
+// Shapes the solver expects of the ten inputs (aq_synth_build order).
+inline std::vector<std::vector<int64_t>> input_shapes(int64_t o, int64_t v) {
+    return {{v, o}, {v, v}, {o, o}, {v, v, v, v}, {v, v, v, o}, {v, v, o, o}, {v, o, v, o}, {v, o, o, v},
+            {v, o, o, o}, {o, o, o, o}};
+}
 
 }  // namespace aqamd
 
 using namespace aqamd;
+
+size_t job_bytes(int o, int v, int nQ, int diis_order, int precision, bool integrals);
 
 struct aq_server {
     int device = 0;
@@ -79,20 +92,42 @@ struct aq_server {
         const int nQ = s.nQ > 0 ? s.nQ : 3 * (s.o + s.v);
         ctx->budget = job.res.bytes_reserved;
         ctx->peak_bytes = ctx_in_use(ctx);
-        std::vector<double> eps(size_t(s.o + s.v)), B(size_t(nQ) * (s.o + s.v) * (s.o + s.v));
-        tensor::check(aq_synth_host(s.seed, s.o, s.v, nQ, s.sigma, eps.data(), B.data()));
-        aq_tensor* ints[10] = {};
+        aq_tensor* ints[10] = {};        // owned by the job (synthetic or FCIDUMP)
+        aq_tensor* const* use = ints;    // what the solver reads
         aq_rhfccsd* cc = nullptr;
+        aq_tensor* mo = nullptr;
         auto cleanup = [&]() {
             if (cc) aq_rhfccsd_destroy(cc);
+            if (mo) aq_tensor_destroy(mo);
             for (auto* t : ints)
                 if (t) aq_tensor_destroy(t);
             aq_ctx_trim(ctx);
         };
         try {
-            tensor::check(aq_synth_build(ctx, s.o, s.v, nQ, eps.data(), B.data(), ints));
-            std::vector<double>().swap(B);
-            tensor::check(aq_rhfccsd_create(ctx, ints, ints + 3, &s.cfg, &cc));
+            if (!job.inputs.empty()) {
+                use = job.inputs.data();
+            } else if (!job.fcidump.empty()) {
+                // the reference's fcidump task: read the file, build the
+                // closed-shell Fock matrix and the Dirac blocks on the device
+                int norb = 0, nelec = 0, ms2 = 0;
+                tensor::check(aq_fcidump_header(job.fcidump.c_str(), &norb, &nelec, &ms2));
+                const int64_t n = norb;
+                std::vector<double> h(size_t(n * n)), eri(size_t(n * n * n * n));
+                double ecore = 0.0, escf = 0.0;
+                tensor::check(aq_fcidump_read(job.fcidump.c_str(), 0, h.data(), eri.data(), &ecore));
+                const int64_t len[4] = {n, n, n, n};
+                tensor::check(aq_tensor_create(ctx, 4, len, 0, &mo));
+                tensor::check(aq_write_all(ctx, mo, eri.data()));
+                std::vector<double>().swap(eri);
+                tensor::check(aq_mo_rhf_blocks(ctx, mo, nelec / 2, h.data(), &escf, nullptr, ints));
+                aq_tensor_destroy(mo);
+                mo = nullptr;
+            } else {
+                std::vector<double> eps(size_t(s.o + s.v)), B(size_t(nQ) * (s.o + s.v) * (s.o + s.v));
+                tensor::check(aq_synth_host(s.seed, s.o, s.v, nQ, s.sigma, eps.data(), B.data()));
+                tensor::check(aq_synth_build(ctx, s.o, s.v, nQ, eps.data(), B.data(), ints));
+            }
+            tensor::check(aq_rhfccsd_create(ctx, use, use + 3, &s.cfg, &cc));
             job.res.mp2 = 0.0;
             tensor::check(aq_rhfccsd_mp2(cc, &job.res.mp2));
             struct Count {
@@ -174,7 +209,18 @@ struct aq_server {
 extern "C" {
 
 size_t aq_job_bytes_estimate(int o, int v, int nQ, int diis_order, int precision) {
+    return job_bytes(o, v, nQ, diis_order, precision, true);
+}
+
+}  // extern "C"
+
+// Device bytes of one job.  integrals = false: the ten input tensors are the
+// caller's (not counted).  nQ < 0: the integrals come from an FCIDUMP file of
+// -nQ orbitals (the full MO tensor is on the device while the blocks are cut).
+size_t job_bytes(int o, int v, int nQ, int diis_order, int precision, bool integrals) {
     if (o <= 0 || v <= 0) return 0;
+    const bool from_file = nQ < 0;
+    const double n4 = from_file ? std::pow(double(-nQ), 4) : 0.0;
     if (nQ <= 0) nQ = 3 * (o + v);
     const double O = o, V = v, Q = nQ, k = std::max(diis_order, 1);
     const double V4 = V * V * V * V, V3 = V * V * V * O, N2 = O * O * V * V, O3V = O * O * O * V, O4 = O * O * O * O;
@@ -185,10 +231,12 @@ size_t aq_job_bytes_estimate(int o, int v, int nQ, int diis_order, int precision
     const double cache64 = 3 * V3 + 12 * N2;
     const double cacheR = precision != AQ_PREC_FP64 ? 0.5 * (V4 + 3 * V3 + 12 * N2) : 0.0;
     const double temps = 3 * N2;
-    const double solve = ints + solver + diis + cache64 + cacheR + temps;
-    const double build = ints + V4 + (V * V + V * O + O * O) * Q;
+    const double solve = (integrals ? ints : 0.0) + solver + diis + cache64 + cacheR + temps;
+    const double build = !integrals ? 0.0 : from_file ? ints + n4 : ints + V4 + (V * V + V * O + O * O) * Q;
     return size_t(8.0 * std::max(solve, build) * 1.10) + (size_t(64) << 20);
 }
+
+extern "C" {
 
 int aq_server_create(int device, size_t hbm_budget, int slots, double max_bypass_s, aq_server** out) {
     return guarded([&] {
@@ -219,16 +267,54 @@ int aq_server_create(int device, size_t hbm_budget, int slots, double max_bypass
 int aq_server_submit(aq_server* s, const aq_job_spec* spec, int64_t* job_id) {
     return guarded([&] {
         AQ_CHECK(s && spec && job_id, AQ_ERR_INVALID, "null argument");
-        AQ_CHECK(spec->o > 0 && spec->v > 0, AQ_ERR_INVALID, "job needs o > 0 and v > 0");
-        const size_t need = aq_job_bytes_estimate(spec->o, spec->v, spec->nQ, spec->cfg.diis_order,
-                                                  spec->cfg.precision);
+        AQ_CHECK(!(spec->inputs && spec->fcidump), AQ_ERR_INVALID, "a job takes inputs or an FCIDUMP path, not both");
+        aq_job_spec js = *spec;
+        std::vector<aq_tensor*> inputs;
+        std::string fcidump;
+        size_t need = 0;
+        if (spec->inputs) {
+            for (int k = 0; k < 10; ++k) {
+                AQ_CHECK(spec->inputs[k], AQ_ERR_INVALID, "job input tensor missing");
+                AQ_CHECK(spec->inputs[k]->ctx && spec->inputs[k]->ctx->device == s->device, AQ_ERR_INVALID,
+                         "job input tensors must live on the server's device");
+                inputs.push_back(spec->inputs[k]);
+            }
+            AQ_CHECK(inputs[0]->ndim == 2, AQ_ERR_INVALID, "fAI must be [v,o]");
+            js.o = int(inputs[0]->len[1]);
+            js.v = int(inputs[0]->len[0]);
+            const auto shapes = input_shapes(js.o, js.v);
+            for (int k = 0; k < 10; ++k)
+                AQ_CHECK(inputs[size_t(k)]->len == shapes[size_t(k)], AQ_ERR_INVALID,
+                         "job input tensor " + std::to_string(k) + " has the wrong shape");
+            AQ_CHECK(js.o > 0 && js.v > 0, AQ_ERR_INVALID, "job needs o > 0 and v > 0");
+            // the integrals are borrowed: reserve the solver's own memory only
+            need = job_bytes(js.o, js.v, 0, js.cfg.diis_order, js.cfg.precision, /*integrals=*/false);
+            // their writes must be complete before another stream reads them
+            for (auto* t : inputs) AQ_HIP(hipStreamSynchronize(t->ctx->stream));
+        } else if (spec->fcidump) {
+            fcidump = spec->fcidump;
+            int norb = 0, nelec = 0, ms2 = 0;
+            tensor::check(aq_fcidump_header(fcidump.c_str(), &norb, &nelec, &ms2));
+            AQ_CHECK(ms2 == 0 && nelec % 2 == 0 && nelec > 0 && 2 * norb > nelec, AQ_ERR_UNSUPPORTED,
+                     "FCIDUMP job needs a closed shell (MS2 = 0, even NELEC < 2 NORB)");
+            js.o = nelec / 2;
+            js.v = norb - js.o;
+            need = job_bytes(js.o, js.v, -norb, js.cfg.diis_order, js.cfg.precision, true);
+        } else {
+            AQ_CHECK(spec->o > 0 && spec->v > 0, AQ_ERR_INVALID, "job needs o > 0 and v > 0");
+            need = aq_job_bytes_estimate(spec->o, spec->v, spec->nQ, spec->cfg.diis_order, spec->cfg.precision);
+        }
         std::lock_guard<std::mutex> lk(s->mu);
         AQ_CHECK(!s->stopping, AQ_ERR_INVALID, "server is stopping");
         AQ_CHECK(need <= s->budget, AQ_ERR_OOM,
                  "job needs " + std::to_string(need) + " B, more than the server budget " + std::to_string(s->budget));
         const int64_t id = s->next_id++;
         Job& j = s->jobs[id];
-        j.spec = *spec;
+        j.spec = js;
+        j.inputs = std::move(inputs);
+        j.fcidump = std::move(fcidump);
+        j.spec.inputs = j.inputs.empty() ? nullptr : j.inputs.data();
+        j.spec.fcidump = j.fcidump.empty() ? nullptr : j.fcidump.c_str();
         j.res.id = id;
         j.res.status = AQ_JOB_QUEUED;
         j.res.slot = -1;

@@ -208,7 +208,7 @@ def serve_stream(a, seed):
                       "seed": seed, "convergence": a.convergence, "diis_order": a.diis,
                       "graph": a.graph, "graph_max_o": a.graph_max_o, "grouping_us": a.grouping_us},
            "grouped_launches": grouping,
-           "gemms_per_grouped_launch": (grouping[1] / grouping[0]) if grouping[0] else None,
+           "gemms_per_grouped_launch": (grouping["gemms"] / grouping["launches"]) if grouping["launches"] else None,
            "jobs": len(res), "converged": len(ok),
            "failed": sum(r["status"] == aq.JOB_FAILED for r in res),
            "first_error": next((r["error"] for r in res if r["status"] == aq.JOB_FAILED), ""),

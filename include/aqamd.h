@@ -78,7 +78,8 @@ int aq_ctx_get_precision(const aq_ctx* ctx, int* precision);
 /* Kernel variant of the large-tile GEMM for one arithmetic (AQ_PREC_*):
  * -1 = library default (or the autotuner's choice), FP64/FP32: 0 = BK 16,
  * 1 = BK 32, 2 = BK 16 with two waves per SIMD; BF16x3: 0 = 128x128 blocks,
- * 1 = 256x128, 2 = 128x256, 3 = 128x128 loading two k-tiles ahead, 4 = 256x256.
+ * 1 = 256x128, 2 = 128x256, 3 = 256x256 with eight waves (two per SIMD),
+ * 4 = 256x256 with four waves.
  * A tuning knob: every variant computes identical results. */
 int aq_ctx_set_gemm_variant(aq_ctx* ctx, int precision, int variant);
 /* GEMM autotuning (no reference counterpart).  With enable != 0 the first
@@ -132,8 +133,20 @@ int aq_tensor_destroy(aq_tensor* t);
 int aq_tensor_ndim(const aq_tensor* t, int* ndim);
 int aq_tensor_lengths(const aq_tensor* t, int64_t* len);
 int aq_tensor_size(const aq_tensor* t, int64_t* size);
-/* Device address of the column-major data (for zero-copy interop). */
+/* Device address of the column-major data (for zero-copy interop).  Reading
+ * through it is always safe once the context's queued work is done
+ * (aq_ctx_sync).  After writing through it, call aq_tensor_touch before the
+ * tensor is next used by the library. */
 int aq_tensor_data(const aq_tensor* t, double** dptr);
+/* Declare that X's data was changed outside the library (through the pointer
+ * of aq_tensor_data): bumps X's write version, the key of every copy the
+ * library derived from it (permuted, FP32 or BF16 operand copies in the
+ * context's cache; a solver's denominators, spin-adapted blocks and packed
+ * ladder operators when X is one of its inputs), so each is rebuilt before its
+ * next use.  Without it those copies keep the old contents.  The counterpart
+ * of writing a CTFTensor through its own write calls (ctf_tensor.hpp:139-247),
+ * which the library's aq_write / aq_write_all already cover. */
+int aq_tensor_touch(aq_tensor* X);
 
 /* ---- operators (CTFTensor methods) ------------------------------------- */
 
@@ -246,7 +259,11 @@ int aq_rhfccsd_default_config(aq_rhfccsd_config* cfg);
  * VABCI[v,v,v,o], VABIJ[v,v,o,o], VAIBJ[v,o,v,o], VAIJB[v,o,o,v],
  * VAIJK[v,o,o,o], VIJKL[o,o,o,o] (layout of rhfaomoints.cxx:58-65), builds the
  * MP2 guess and the DIIS state.  The inputs are borrowed (not copied) and must
- * outlive the solver and stay unmodified. */
+ * outlive the solver.  A caller that changes an input between iterations
+ * (aq_write*, or a write through aq_tensor_data followed by aq_tensor_touch)
+ * gets the next iteration computed from the new values: the solver rebuilds
+ * the denominators, spin-adapted blocks and packed ladder operators it derived
+ * from a changed input; the amplitudes and DIIS history are kept. */
 int aq_rhfccsd_create(aq_ctx* ctx, const aq_tensor* const* fock3, const aq_tensor* const* v7,
                       const aq_rhfccsd_config* cfg, aq_rhfccsd** out);
 int aq_rhfccsd_destroy(aq_rhfccsd* cc);
@@ -282,6 +299,27 @@ int aq_rhfccsd_graph_stats(const aq_rhfccsd* cc, int* captures, int* replays, in
 /* 1 when the solver evaluates the ladder on packed pairs (see
  * aq_rhfccsd_config.packed_ladder), 0 when it runs the full contraction. */
 int aq_rhfccsd_packed_ladder(const aq_rhfccsd* cc, int* packed);
+
+/* ---- DIIS (DIIS<T>, src/convergence/diis.hpp:36-227) -------------------
+ *
+ * The Pulay extrapolator the CCSD drivers use, over device-resident history:
+ * aq_diis_extrapolate(x, dx) stores copies of the nx tensors x and the ndx
+ * residuals dx as the newest history slot, builds the bordered system from the
+ * inner products of dx with every stored residual, solves it (a retry with
+ * eps*B[n-1][n-1] added to the diagonal when a pivot is exactly zero, the
+ * dsysv 'info > 0' path of diis.hpp:171-187; AQ_ERR_NUMERIC if that fails
+ * too) and overwrites x and dx with the extrapolated combinations.  `start`
+ * and `damping` as DIIS::start/damping (diis.hpp:151-164).  Any order >= 0
+ * (history memory: order copies of every x and dx).  The tensors must keep
+ * their sizes from call to call. */
+typedef struct aq_diis aq_diis;
+int aq_diis_create(aq_ctx* ctx, int order, int start, double damping, int nx, int ndx, aq_diis** out);
+int aq_diis_destroy(aq_diis* d);
+int aq_diis_extrapolate(aq_diis* d, aq_tensor* const* x, aq_tensor* const* dx);
+/* The coefficients of the last solve (order + 1 values, the last one the
+ * Lagrange multiplier; up to `cap` copied, *n receives the count) and whether
+ * that solve took the eps retry. */
+int aq_diis_coefficients(const aq_diis* d, double* c, int cap, int* n, int* used_eps_retry);
 
 /* ---- synthetic closed-shell integrals (workload construction) ---------- */
 
@@ -447,6 +485,21 @@ typedef struct aq_job_spec {
     int o, v, nQ;           /* nQ <= 0: 3*(o+v) */
     double sigma;           /* <= 0: aq_synth_default_sigma */
     aq_rhfccsd_config cfg;  /* solver options */
+    /* Caller-supplied integrals instead of the synthetic system (at most one
+     * of the two; both null = synthetic):
+     *  - inputs: the ten tensors aq_rhfccsd_create takes, in the order of
+     *    aq_synth_build (fAI, fAB, fIJ, VABCD, VABCI, VABIJ, VAIBJ, VAIJB,
+     *    VAIJK, VIJKL), e.g. from aq_aomo_rhf (the reference's aomoints task
+     *    feeding ccsd, rhfaomoints.cxx:38-180).  Tensors of any context on the
+     *    server's device; submit waits for their context's queued work; they
+     *    are borrowed, not counted against the server budget, and must stay
+     *    unchanged until the job is done.  o and v are taken from fAI.
+     *  - fcidump: path of an FCIDUMP file that the job's slot reads and turns
+     *    into the ten tensors (aq_fcidump_read + aq_mo_rhf_blocks: the
+     *    reference's fcidump task, fcidump.cxx:43-517); closed shell (MS2 = 0),
+     *    o = NELEC/2, v = NORB - o.  The path is copied at submit. */
+    aq_tensor* const* inputs;
+    const char* fcidump;
 } aq_job_spec;
 
 enum { AQ_JOB_QUEUED = 0, AQ_JOB_RUNNING = 1, AQ_JOB_DONE = 2, AQ_JOB_FAILED = 3 };

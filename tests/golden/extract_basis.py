@@ -1,9 +1,10 @@
-"""Writes the GENBAS-format basis-set data files next to this script: the
-entries for H, C, N and O of cc-pVDZ and DZ (Dunning), taken verbatim from the
-EMSL-derived library files that Aquarius distributes under basis/.  Run once,
-in a checkout that has the reference tree at /root/reference:
+"""Data recipe (not shipped code): writes the GENBAS-format basis-set files
+aquarius_amd/basis/{cc-pVDZ,DZ}.gbs, the entries for H, C, N and O of cc-pVDZ
+and DZ (Dunning), taken verbatim from the EMSL-derived library files that
+Aquarius distributes under basis/.  Run once, in a checkout that has the
+reference tree at /root/reference:
 
-    python aquarius_amd/basis/extract.py
+    python tests/golden/extract_basis.py
 """
 
 import os
@@ -32,7 +33,7 @@ def entries(text):
 
 
 def main():
-    here = os.path.dirname(os.path.abspath(__file__))
+    here = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "..", "aquarius_amd", "basis")
     for s in SETS:
         with open(os.path.join(SRC, s)) as f:
             head, ent = entries(f.read())

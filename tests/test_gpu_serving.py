@@ -121,10 +121,10 @@ def test_oversize_rejected_and_failures_isolated():
         with pytest.raises(aq.AqError) as e:
             s.submit(1, 20, 140, config=cfg("fp64"))
         assert e.value.code == 3
-        bad = s.submit(2, 3, 12, config=aq.default_config(diis_order=9))   # unsupported DIIS order
+        bad = s.submit(2, 3, 12, config=aq.default_config(diis_order=-1))   # invalid DIIS order
         good = s.submit(3, 3, 12, config=cfg("fp64"))
         rb, rg = s.wait(bad), s.wait(good)
         st = s.stats()
-    assert rb["status"] == aq.JOB_FAILED and rb["error_code"] == 4 and "DIIS order" in rb["error"]
+    assert rb["status"] == aq.JOB_FAILED and rb["error_code"] == 1 and "DIIS order" in rb["error"]
     assert rg["status"] == aq.JOB_DONE and rg["converged"]
     assert st["completed"] == 1 and st["failed"] == 1 and st["reserved"] == 0
