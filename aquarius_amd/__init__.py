@@ -80,6 +80,7 @@ _SIGS = {
     "aq_fill": [_P, _P, C.c_double, C.c_double],
     "aq_dot": [_P, _P, C.c_char_p, _P, C.c_char_p, _DP],
     "aq_weight": [_P, _P, C.c_int, C.POINTER(_DP), C.c_double],
+    "aq_weight_noalloc": [_P, _P, C.c_int, C.POINTER(_DP), C.c_double],
     "aq_norm": [_P, _P, C.c_int, _DP],
     "aq_read": [_P, _P, C.c_int64, _I64P, _DP],
     "aq_write": [_P, _P, C.c_int64, _I64P, _DP, C.c_double, C.c_double],

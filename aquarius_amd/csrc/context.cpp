@@ -207,6 +207,7 @@ int aq_ctx_create(int device, size_t hbm_budget, aq_ctx** out) {
         AQ_HIP(hipGetDeviceCount(&ndev));
         AQ_CHECK(device >= 0 && device < ndev, AQ_ERR_INVALID, "no such HIP device");
         AQ_HIP(hipSetDevice(device));
+        noalloc_scratch_init(device);
         {
             // keep memory released by hipFreeAsync in the device pool instead of
             // returning it to the driver at every synchronisation

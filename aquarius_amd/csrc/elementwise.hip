@@ -289,7 +289,7 @@ __global__ __launch_bounds__(TPB) void k_weight(double* x, int64_t n, WeightArgs
 }
 
 void launch_weight(aq_ctx* ctx, double* x, int ndim, const int64_t* len, const double* d_dev, const int64_t* d_off,
-                   double shift) {
+                   double shift, bool timed) {
     WeightArgs w;
     w.nd = ndim;
     int64_t n = 1;
@@ -299,6 +299,11 @@ void launch_weight(aq_ctx* ctx, double* x, int ndim, const int64_t* len, const d
         if (k < ndim) n *= len[k];
     }
     if (n == 0) return;
+    if (!timed) {   // no event objects: the allocation-free entry point
+        hipLaunchKernelGGL(k_weight, dim3(grid_for(n, TPB * 4)), dim3(TPB), 0, ctx->stream, x, n, w, d_dev, shift);
+        AQ_HIP(hipGetLastError());
+        return;
+    }
     TimedLaunch tl(ctx, "weight", 0.0, 16.0 * n);
     hipLaunchKernelGGL(k_weight, dim3(grid_for(n, TPB * 4)), dim3(TPB), 0, ctx->stream, x, n, w, d_dev, shift);
     AQ_HIP(hipGetLastError());

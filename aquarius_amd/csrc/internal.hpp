@@ -175,6 +175,7 @@ void tensor_free(aq_ctx* ctx, double* p, size_t bytes);
 size_t ctx_in_use(const aq_ctx* ctx);
 void evict_cache(aq_ctx* ctx, size_t need);
 void invalidate_cache(aq_ctx* ctx);                  // every cached copy must be rebuilt before use
+void noalloc_scratch_init(int device);               // noalloc.cpp: resolve the fixed device array once per device
 
 struct Scratch {  // RAII pooled device buffer
     aq_ctx* ctx;
@@ -223,7 +224,7 @@ void launch_strided_convert(aq_ctx* ctx, const StridedPlan& p, const double* src
                             int64_t plane_stride);
 void launch_div(aq_ctx* ctx, const double* A, const double* B, double* C, int64_t n, double alpha, double beta);
 void launch_weight(aq_ctx* ctx, double* x, int ndim, const int64_t* len, const double* d_dev,
-                   const int64_t* d_off, double shift);
+                   const int64_t* d_off, double shift, bool timed = true);
 // deterministic reductions into ctx->h_result[slot]; caller syncs
 void launch_dot(aq_ctx* ctx, const double* a, const double* b, int64_t n, int slot);
 void launch_norm(aq_ctx* ctx, const double* x, int64_t n, int p, int slot);

@@ -177,6 +177,20 @@ int aq_dot(aq_ctx* ctx, const aq_tensor* A, const char* idxA,
  * doubles. */
 int aq_weight(aq_ctx* ctx, aq_tensor* X, int ndim, const double* const* d, double shift);
 
+/* aq_weight without any allocation: no reference counterpart; for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments, semantics and
+ * results as aq_weight (bitwise), but the denominators are staged in one fixed
+ * device array per GPU (AQ_NOALLOC_WEIGHT_MAX doubles, held until the process
+ * ends) instead of pooled device memory, and no timing events are recorded.
+ * The lengths of X must sum to at most AQ_NOALLOC_WEIGHT_MAX, otherwise
+ * AQ_ERR_INVALID.  Calls on one GPU share that array: they take a per-device
+ * lock and return only after the kernel has finished (synchronising), so calls
+ * from several threads or contexts serialise instead of overwriting each
+ * other's denominators. */
+#define AQ_NOALLOC_WEIGHT_MAX 65536
+int aq_weight_noalloc(aq_ctx* ctx, aq_tensor* X, int ndim, const double* const* d, double shift);
+
 /* p = 0: max|x|, p = 1: sum|x|, p = 2: 2-norm.  CTFTensor::norm
  * (ctf_tensor.cxx:403-431).  Synchronising. */
 int aq_norm(aq_ctx* ctx, const aq_tensor* X, int p, double* out);
