@@ -347,9 +347,23 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
     // epilogue: lane holds C[m = mb + lane%16][n = nb + drow(lane, r)]
     double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
     const int64_t ldc = g.P ? g.M : g.ldc;
+    // per row of blocks, every C value is read before any is written: one
+    // memory round trip per row instead of one per element (a load after a
+    // store to C could alias it, so the compiler keeps that order)
+    const bool addc = !g.P && g.beta != 0.0;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int64_t m = m0 + wm * WTM + i * 16 + lane % 16;
+        double cv[FN][4];
+        if (addc) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t n = n0 + wn * WTN + j * 16 + Mfma<T>::drow(lane, r);
+                    cv[j][r] = (m < g.M && n < g.N) ? Cb[m + n * ldc] : 0.0;
+                }
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
@@ -357,7 +371,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
                 const int64_t n = n0 + wn * WTN + j * 16 + Mfma<T>::drow(lane, r);
                 if (m < g.M && n < g.N) {
                     double v = g.alpha * double(acc[i][j][r]);
-                    if (!g.P && g.beta != 0.0) v += g.beta * Cb[m + n * ldc];
+                    if (addc) v += g.beta * cv[j][r];
                     Cb[m + n * ldc] = v;
                 }
             }
@@ -374,11 +388,15 @@ __global__ __launch_bounds__(256, MINW) void k_gemm(GemmArgs g) {
 
 // NT = 64*WM*WN threads (4 waves: one per SIMD; 8 waves: two per SIMD, each
 // holding half the accumulators, so one wave's LDS and barrier waits are
-// covered by the other's MFMAs).
-template <int BM, int BN, int WM, int WN, bool REMAP>
+// covered by the other's MFMAs).  DIRECT: interior tiles load their operand
+// tiles straight from global memory into LDS (global_load_lds_dwordx4), with
+// no staging registers and no LDS store instructions; edge tiles, and every
+// tile of a DIRECT = false instantiation, stage through registers.
+template <int BM, int BN, int WM, int WN, bool REMAP, bool DIRECT>
 __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, int bz) {
     constexpr int BK = 32;
     constexpr int NT = 64 * WM * WN;
+    constexpr int NW = WM * WN;
     // LDS rows of BK bf16 (64 bytes, four 16-byte chunks), chunk c of row r
     // stored at chunk c ^ ((r >> 1) & 3).  Measured on gfx950 (bank-conflict
     // counter): this layout serves both the MFMA operand reads (lane -> row
@@ -392,11 +410,18 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     constexpr int VA = BM * BK / 8, VB = BN * BK / 8;   // 8-element vectors per plane
     constexpr int NA = (VA + NT - 1) / NT, NB = (VB + NT - 1) / NT;
     static_assert(WM * WN == 4 || WM * WN == 8, "4 or 8 waves");
+    static_assert(!DIRECT || ((BM / 16) % NW == 0 && (BN / 16) % NW == 0), "direct loads: whole 16-row groups per wave");
 
+    // one LDS object per buffer, so that the compiler can tell the buffer a
+    // direct load fills from the one the MFMAs read
     __shared__ i16x8 dummy_align;                   // keeps LDS 16-byte aligned
-    __shared__ __attribute__((aligned(16))) unsigned short As[2][2][BM][KP];   // [buf][hi/lo][m][k]
-    __shared__ __attribute__((aligned(16))) unsigned short Bs[2][2][BN][KP];
+    __shared__ __attribute__((aligned(16))) unsigned short As0[2][BM][KP];   // [hi/lo][m][k]
+    __shared__ __attribute__((aligned(16))) unsigned short As1[2][BM][KP];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs0[2][BN][KP];
+    __shared__ __attribute__((aligned(16))) unsigned short Bs1[2][BN][KP];
     (void)dummy_align;
+    auto As = [&](int buf) -> unsigned short (*)[BM][KP] { return buf ? As1 : As0; };
+    auto Bs = [&](int buf) -> unsigned short (*)[BN][KP] { return buf ? Bs1 : Bs0; };
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
@@ -447,21 +472,23 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
         }
     };
     auto store = [&](int buf, const i16x8 (&xa)[2][NA], const i16x8 (&xb)[2][NB]) {
+        auto* as = As(buf);
+        auto* bs = Bs(buf);
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
             const int e = tid + NT * r;
             if (e >= VA) continue;
             const int kk = (e % (BK / 8)) * 8, mm = e / (BK / 8);
-            *reinterpret_cast<i16x8*>(&As[buf][0][mm][swz(mm, kk)]) = xa[0][r];
-            *reinterpret_cast<i16x8*>(&As[buf][1][mm][swz(mm, kk)]) = xa[1][r];
+            *reinterpret_cast<i16x8*>(&as[0][mm][swz(mm, kk)]) = xa[0][r];
+            *reinterpret_cast<i16x8*>(&as[1][mm][swz(mm, kk)]) = xa[1][r];
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
             const int e = tid + NT * r;
             if (e >= VB) continue;
             const int kk = (e % (BK / 8)) * 8, nn = e / (BK / 8);
-            *reinterpret_cast<i16x8*>(&Bs[buf][0][nn][swz(nn, kk)]) = xb[0][r];
-            *reinterpret_cast<i16x8*>(&Bs[buf][1][nn][swz(nn, kk)]) = xb[1][r];
+            *reinterpret_cast<i16x8*>(&bs[0][nn][swz(nn, kk)]) = xb[0][r];
+            *reinterpret_cast<i16x8*>(&bs[1][nn][swz(nn, kk)]) = xb[1][r];
         }
     };
 
@@ -473,64 +500,158 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
 
     // MFMA operands: src0 (16 x 32) from the B tile, lane holds row lane%16,
     // k = 8*(lane/16) .. +7; src1 (32 x 16) from the A tile, lane holds column
-    // lane%16, the same k.
+    // lane%16, the same k.  One k-tile of MFMAs: the B fragments first, then
+    // the A fragments of row i+1 are read while row i's MFMAs run (two
+    // register pairs), so the MFMAs never wait for an LDS read they need next;
+    // `mid` runs halfway.  The MFMA sequence of every accumulator (k-tiles in
+    // order, three products per block in the order lo*hi, hi*lo, hi*hi) is the
+    // same on every path and in every variant.
+    const int kq = 8 * (lane / 16);
+    auto mfma_tile = [&](const unsigned short (*as)[BM][KP], const unsigned short (*bs)[BN][KP], auto&& mid) {
+        bf16x8 xh[FN], xl[FN], yh[2], yl[2];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) {
+            const int nn = wn * WTN + j * 16 + lane % 16;
+            xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&bs[0][nn][swz(nn, kq)]));
+            xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&bs[1][nn][swz(nn, kq)]));
+        }
+        auto read_a = [&](int i, int slot) {
+            const int mm = wm * WTM + i * 16 + lane % 16;
+            yh[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&as[0][mm][swz(mm, kq)]));
+            yl[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&as[1][mm][swz(mm, kq)]));
+        };
+        read_a(0, 0);
+#pragma unroll
+        for (int i = 0; i < FM; ++i) {
+            if (i + 1 < FM) read_a(i + 1, (i + 1) & 1);
+            __builtin_amdgcn_sched_barrier(0);   // keep those reads ahead of row i's MFMAs
+            if (i == FM / 2) mid();
+#pragma unroll
+            for (int j = 0; j < FN; ++j) {
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i & 1], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i & 1], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i & 1], acc[i][j], 0, 0, 0);
+            }
+        }
+    };
+
     const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
-    i16x8 ra0[2][NA], rb0[2][NB];
-    {
+    if (DIRECT && m0 + BM <= g.M && n0 + BN <= g.N) {
+        // Each wave instruction moves 64 lanes x 16 bytes into 1024 contiguous
+        // LDS bytes: 16 rows of one plane.  Lane l fills row l/4 of the group
+        // at physical chunk l%4, which holds the logical k-chunk
+        // (l%4) ^ ((row >> 1) & 3) of the swizzled layout.  A chunk starting
+        // at or beyond kend reads an in-bounds address and is zeroed in LDS
+        // once its load has landed (only in the partial last k-tile).
+        constexpr int GA = BM / 16 / NW, GB = BN / 16 / NW;   // row groups per wave and plane
+        const int rl = lane >> 2, pc = lane & 3;
+        auto issue = [&](auto S, int64_t k0) {
+            constexpr int s = decltype(S)::value;
+            auto* as = s ? As1 : As0;
+            auto* bs = s ? Bs1 : Bs0;
+#pragma unroll
+            for (int q = 0; q < GA; ++q) {
+                const int row = (wave + NW * q) * 16 + rl;
+                const int64_t k = k0 + 8 * (pc ^ ((row >> 1) & 3));
+                const unsigned short* src = A + (k < kend ? k : kbeg) + (m0 + row) * g.lda;
+                __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&as[0][(wave + NW * q) * 16][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(src + g.planeA, (__attribute__((address_space(3))) void*)&as[1][(wave + NW * q) * 16][0], 16, 0, 0);
+            }
+#pragma unroll
+            for (int q = 0; q < GB; ++q) {
+                const int row = (wave + NW * q) * 16 + rl;
+                const int64_t k = k0 + 8 * (pc ^ ((row >> 1) & 3));
+                const unsigned short* src = B + (k < kend ? k : kbeg) + (n0 + row) * g.ldb;
+                __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&bs[0][(wave + NW * q) * 16][0], 16, 0, 0);
+                __builtin_amdgcn_global_load_lds(src + g.planeB, (__attribute__((address_space(3))) void*)&bs[1][(wave + NW * q) * 16][0], 16, 0, 0);
+            }
+        };
+        // wait for this wave's direct loads; in a partial k-tile, zero the
+        // chunks past kend (each lane its own chunk, after its load landed)
+        auto land = [&](auto S, int64_t k0) {
+            constexpr int s = decltype(S)::value;
+            __builtin_amdgcn_s_waitcnt(0);
+            if (k0 + BK > kend) {
+                auto* as = s ? As1 : As0;
+                auto* bs = s ? Bs1 : Bs0;
+                const i16x8 z = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+                for (int q = 0; q < GA; ++q) {
+                    const int row = (wave + NW * q) * 16 + rl;
+                    if (k0 + 8 * (pc ^ ((row >> 1) & 3)) >= kend) {
+                        *reinterpret_cast<i16x8*>(&as[0][row][8 * pc]) = z;
+                        *reinterpret_cast<i16x8*>(&as[1][row][8 * pc]) = z;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < GB; ++q) {
+                    const int row = (wave + NW * q) * 16 + rl;
+                    if (k0 + 8 * (pc ^ ((row >> 1) & 3)) >= kend) {
+                        *reinterpret_cast<i16x8*>(&bs[0][row][8 * pc]) = z;
+                        *reinterpret_cast<i16x8*>(&bs[1][row][8 * pc]) = z;
+                    }
+                }
+            }
+        };
+        auto step = [&](auto S, int64_t kt) {
+            constexpr int s = decltype(S)::value;
+            const int64_t k1 = kbeg + (kt + 1) * BK;
+            if (kt + 1 < nk) issue(std::integral_constant<int, 1 - s>{}, k1);
+            mfma_tile(s ? As1 : As0, s ? Bs1 : Bs0, [] {});
+            if (kt + 1 < nk) land(std::integral_constant<int, 1 - s>{}, k1);
+            __syncthreads();
+        };
+        if (nk > 0) {
+            issue(std::integral_constant<int, 0>{}, kbeg);
+            land(std::integral_constant<int, 0>{}, kbeg);
+            __syncthreads();
+        }
+        for (int64_t kt = 0; kt < nk; kt += 2) {
+            step(std::integral_constant<int, 0>{}, kt);
+            if (kt + 1 < nk) step(std::integral_constant<int, 1>{}, kt + 1);
+        }
+    } else {
         // One barrier per k-tile.  Tile kt+1 (in registers) is stored to the
         // other LDS buffer halfway through tile kt's MFMAs, and the loads of
         // tile kt+2 are issued right after that store, so a global load has a
-        // whole k-tile of MFMA work to arrive.  Inside a tile the A fragments
-        // of row i+1 are read while row i's MFMAs run (two register pairs),
-        // so the MFMAs never wait for an LDS read they need next.
-        // The MFMA sequence of every accumulator (k-tiles in order, three
-        // products per block in the order lo*hi, hi*lo, hi*hi) is the one
-        // every other BF16x3 path uses.
+        // whole k-tile of MFMA work to arrive.
+        i16x8 ra0[2][NA], rb0[2][NB];
         if (nk > 0) {
             load(kbeg, ra0, rb0);
             store(0, ra0, rb0);
             if (nk > 1) load(kbeg + BK, ra0, rb0);
             __syncthreads();
         }
-        const int kq = 8 * (lane / 16);
         for (int64_t kt = 0; kt < nk; ++kt) {
             const int buf = int(kt & 1);
-            bf16x8 xh[FN], xl[FN], yh[2], yl[2];
-#pragma unroll
-            for (int j = 0; j < FN; ++j) {
-                const int nn = wn * WTN + j * 16 + lane % 16;
-                xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][0][nn][swz(nn, kq)]));
-                xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&Bs[buf][1][nn][swz(nn, kq)]));
-            }
-            auto read_a = [&](int i, int slot) {
-                const int mm = wm * WTM + i * 16 + lane % 16;
-                yh[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][0][mm][swz(mm, kq)]));
-                yl[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&As[buf][1][mm][swz(mm, kq)]));
-            };
-            read_a(0, 0);
-#pragma unroll
-            for (int i = 0; i < FM; ++i) {
-                if (i + 1 < FM) read_a(i + 1, (i + 1) & 1);
-                if (i == FM / 2 && kt + 1 < nk) {
+            mfma_tile(As(buf), Bs(buf), [&] {
+                if (kt + 1 < nk) {
                     store(buf ^ 1, ra0, rb0);
                     if (kt + 2 < nk) load(kbeg + (kt + 2) * BK, ra0, rb0);
                 }
-#pragma unroll
-                for (int j = 0; j < FN; ++j) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i & 1], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[i & 1], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i & 1], acc[i][j], 0, 0, 0);
-                }
-            }
+            });
             __syncthreads();
         }
     }
 
     double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
     const int64_t ldc = g.P ? g.M : g.ldc;
+    // per row of blocks, every C value is read before any is written (one
+    // memory round trip per row instead of one per element)
+    const bool addc = !g.P && g.beta != 0.0;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int64_t m = m0 + wm * WTM + i * 16 + lane % 16;
+        double cv[FN][4];
+        if (addc) {
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t n = n0 + wn * WTN + j * 16 + 4 * (lane / 16) + r;
+                    cv[j][r] = (m < g.M && n < g.N) ? Cb[m + n * ldc] : 0.0;
+                }
+        }
 #pragma unroll
         for (int j = 0; j < FN; ++j) {
 #pragma unroll
@@ -538,7 +659,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
                 const int64_t n = n0 + wn * WTN + j * 16 + 4 * (lane / 16) + r;
                 if (m < g.M && n < g.N) {
                     double v = g.alpha * double(acc[i][j][r]);
-                    if (!g.P && g.beta != 0.0) v += g.beta * Cb[m + n * ldc];
+                    if (addc) v += g.beta * cv[j][r];
                     Cb[m + n * ldc] = v;
                 }
             }
@@ -546,9 +667,9 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     }
 }
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, bool DIRECT>
 __global__ __launch_bounds__(64 * WM * WN, 1) void k_gemm_bf16x3(GemmArgs g) {
-    gemm_tile_bf16x3<BM, BN, WM, WN, true>(g, blockIdx.x, blockIdx.z);
+    gemm_tile_bf16x3<BM, BN, WM, WN, true, DIRECT>(g, blockIdx.x, blockIdx.z);
 }
 
 // ---------------------------------------------------------- grouped launch
@@ -594,7 +715,7 @@ __global__ __launch_bounds__(256, 1) void k_gemm_bf16x3_grouped(GemmGroup G) {
         const int p = group_problem(G, bid);
         const int64_t tiles = ((G.p[p].M + BM - 1) / BM) * ((G.p[p].N + BN - 1) / BN);
         const int64_t local = bid - G.first[p];
-        gemm_tile_bf16x3<BM, BN, WM, WN, false>(G.p[p], local % tiles, int(local / tiles));
+        gemm_tile_bf16x3<BM, BN, WM, WN, false, false>(G.p[p], local % tiles, int(local / tiles));
         __syncthreads();   // the next tile reuses the LDS buffers
     }
 }
@@ -630,9 +751,9 @@ static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid, int variant) {
 // single wave per SIMD exposes (FP64 35 -> 56 TF/s; FP32 unchanged at ~108).
 int default_gemm_variant(Fmt fmt) { return fmt == FMT_BF16X2 ? 0 : 2; }
 
-template <int BM, int BN, int WM, int WN>
+template <int BM, int BN, int WM, int WN, bool DIRECT = false>
 static void run_cfg_bf16(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
-    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN>), grid, dim3(64 * WM * WN), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_gemm_bf16x3<BM, BN, WM, WN, DIRECT>), grid, dim3(64 * WM * WN), 0, ctx->stream, a);
 }
 
 // Kernel launch for one (format, block shape, variant); the caller checked the shape.
@@ -648,7 +769,7 @@ static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const Ge
         else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
         else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
     } else {
-        if (BM == 256 && BN == 256 && variant == 3) run_cfg_bf16<256, 256, 2, 4>(ctx, a, grid);
+        if (BM == 256 && BN == 256 && variant == 3) run_cfg_bf16<256, 256, 2, 4, true>(ctx, a, grid);
         else if (BM == 256 && BN == 256) run_cfg_bf16<256, 256, 2, 2>(ctx, a, grid);
         else if (BM == 128 && BN == 128) run_cfg_bf16<128, 128, 2, 2>(ctx, a, grid);
         else if (BM == 256) run_cfg_bf16<256, 128, 2, 2>(ctx, a, grid);

@@ -50,15 +50,25 @@ struct aq_rhfccsd {
     int iter = 0;
     int last_prec = AQ_PREC_FP64;
 
+    // Stage of the precision policy reached so far: 0 = the bulk precision,
+    // 1 = the optional FP32 stage, 2 = FP64 refinement.  Stages only advance:
+    // once the residual has dropped below a threshold the remaining iterations
+    // stay at that stage's precision even if DIIS lets conv rise above it again.
+    int policy_stage = 0;
+
+    int stage_for(double c) const {
+        if (cfg.refine_below > 0.0 && c < cfg.refine_below) {
+            // optional FP32 stage between the reduced phase and FP64 refinement
+            if (cfg.fp32_stage_below > 0.0 && c >= cfg.fp32_stage_below && cfg.precision != AQ_PREC_FP32) return 1;
+            return 2;
+        }
+        return 0;
+    }
+
     int next_precision() const {
         if (cfg.precision == AQ_PREC_FP64) return AQ_PREC_FP64;
-        if (cfg.refine_below > 0.0 && conv < cfg.refine_below) {
-            // optional FP32 stage between the reduced phase and FP64 refinement
-            if (cfg.fp32_stage_below > 0.0 && conv >= cfg.fp32_stage_below && cfg.precision != AQ_PREC_FP32)
-                return AQ_PREC_FP32;
-            return AQ_PREC_FP64;
-        }
-        return cfg.precision;
+        const int st = std::max(policy_stage, stage_for(conv));
+        return st == 2 ? AQ_PREC_FP64 : st == 1 ? AQ_PREC_FP32 : cfg.precision;
     }
 
     // Versions of the borrowed inputs that setup derived quantities from:
@@ -376,6 +386,7 @@ struct aq_rhfccsd {
     void iterate() {
         refresh_inputs();
         const int prec = next_precision();
+        if (cfg.precision != AQ_PREC_FP64) policy_stage = std::max(policy_stage, stage_for(conv));
         if (cfg.diis_reset_on_refine && iter > 0 && prec == AQ_PREC_FP64 && last_prec != AQ_PREC_FP64)
             diis.reset(new DIIS(ctx, cfg.diis_order, cfg.diis_start, cfg.diis_damping, 2, 2));
         last_prec = prec;

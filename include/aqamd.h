@@ -230,7 +230,8 @@ typedef struct aq_rhfccsd_config {
     double diis_damping;
     /* contraction arithmetic while conv >= refine_below (AQ_PREC_*) */
     int precision;
-    /* switch to FP64 contractions once conv < refine_below (0 = never) */
+    /* switch to FP64 contractions once conv < refine_below (0 = never); the
+     * switch is final: later iterations stay FP64 even if conv rises again */
     double refine_below;
     /* contractions below this many FLOPs stay FP64 in reduced precision */
     double min_reduced_flops;
