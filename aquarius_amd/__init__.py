@@ -525,7 +525,8 @@ _SIGS.update({
     "aq_server_stats_get": [_P, C.POINTER(ServerStats)],
     "aq_server_destroy": [_P],
     "aq_server_set_grouping": [_P, C.c_int],
-    "aq_server_grouping_stats": [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int)],
+    "aq_server_grouping_stats": [_P, C.POINTER(C.c_int64), C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                 C.POINTER(C.c_int64)],
 })
 _RESTYPE["aq_job_bytes_estimate"] = C.c_size_t
 
@@ -582,9 +583,9 @@ class Server:
         _ck(lib().aq_server_set_grouping(self.h, int(window_us)))
 
     def grouping_stats(self):
-        n, g, mx = C.c_int64(), C.c_int64(), C.c_int()
-        _ck(lib().aq_server_grouping_stats(self.h, C.byref(n), C.byref(g), C.byref(mx)))
-        return {"launches": n.value, "gemms": g.value, "max_group": mx.value}
+        n, g, mx, solo = C.c_int64(), C.c_int64(), C.c_int(), C.c_int64()
+        _ck(lib().aq_server_grouping_stats(self.h, C.byref(n), C.byref(g), C.byref(mx), C.byref(solo)))
+        return {"launches": n.value, "gemms": g.value, "max_group": mx.value, "solo": solo.value}
 
     def close(self):
         if self.h:

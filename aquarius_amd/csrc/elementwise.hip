@@ -128,14 +128,19 @@ __global__ __launch_bounds__(TPB) void k_strided(PlanDev p, const double* __rest
 }
 
 // LDS-tiled transpose: dims d0 (unit stride in dst) and d1 (unit stride in
-// src) are tiled 32x32; the other dims are walked per block.  Reads are
-// coalesced along d1, writes along d0.
-constexpr int TT = 32;
+// src) are tiled T x T; the other dims are walked per block.  Reads are
+// coalesced along d1, writes along d0.  T = 32 for FP64 output (256-byte
+// rows); T = 64 for the BF16 hi/lo planes, so that each plane's row is 128
+// bytes, a whole cache line, instead of 64.
+template <class Out> struct TransposeTile { static constexpr int T = 32; };
+template <> struct TransposeTile<BF16X2Out> { static constexpr int T = 64; };
 
 template <class Out>
 __global__ __launch_bounds__(TPB) void k_transpose(PlanDev p, int d0, int d1, int64_t tiles0, int64_t tiles1,
                                                     const double* __restrict__ src, Out out) {
-    __shared__ double tile[TT][TT + 1];
+    constexpr int T = TransposeTile<Out>::T;
+    constexpr int R = TPB / T;   // rows per pass
+    __shared__ double tile[T][T + 1];
     const int64_t b = blockIdx.x + int64_t(blockIdx.y) * gridDim.x;
     const int64_t nt = tiles0 * tiles1;
     const int64_t t0 = (b % nt) % tiles0, t1 = (b % nt) / tiles0;
@@ -149,18 +154,18 @@ __global__ __launch_bounds__(TPB) void k_transpose(PlanDev p, int d0, int d1, in
         os += c * p.ssrc[k];
     }
     if (rest != 0) return;   // past the last slab
-    const int tx = threadIdx.x % TT, ty = threadIdx.x / TT;   // 32 x 8
+    const int tx = threadIdx.x % T, ty = threadIdx.x / T;
     const int64_t L0 = p.len[d0], L1 = p.len[d1];
     // read: fast along d1 (src unit stride)
 #pragma unroll
-    for (int r = 0; r < TT; r += 8) {
-        const int64_t i1 = t1 * TT + tx, i0 = t0 * TT + ty + r;
+    for (int r = 0; r < T; r += R) {
+        const int64_t i1 = t1 * T + tx, i0 = t0 * T + ty + r;
         if (i0 < L0 && i1 < L1) tile[ty + r][tx] = src[os + i1 * p.ssrc[d1] + i0 * p.ssrc[d0]];
     }
     __syncthreads();
 #pragma unroll
-    for (int r = 0; r < TT; r += 8) {
-        const int64_t i0 = t0 * TT + tx, i1 = t1 * TT + ty + r;
+    for (int r = 0; r < T; r += R) {
+        const int64_t i0 = t0 * T + tx, i1 = t1 * T + ty + r;
         if (i0 < L0 && i1 < L1) out.put(od + i0 * p.sdst[d0] + i1 * p.sdst[d1], tile[tx][ty + r]);
     }
 }
@@ -177,7 +182,8 @@ static void run_strided(aq_ctx* ctx, const StridedPlan& sp, const double* src, c
     }
     TimedLaunch tl(ctx, name, 0.0, bytes);
     if (read_src && p.nr == 0 && d0 >= 0 && d1 >= 0 && d0 != d1 && p.len[d0] >= 8 && p.len[d1] >= 8) {
-        const int64_t tiles0 = (p.len[d0] + TT - 1) / TT, tiles1 = (p.len[d1] + TT - 1) / TT;
+        constexpr int T = TransposeTile<Out>::T;
+        const int64_t tiles0 = (p.len[d0] + T - 1) / T, tiles1 = (p.len[d1] + T - 1) / T;
         const int64_t slabs = p.nout / (p.len[d0] * p.len[d1]);
         const int64_t nb = tiles0 * tiles1 * slabs;
         const int64_t gx = std::min<int64_t>(nb, 65535);

@@ -509,23 +509,39 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     const int kq = 8 * (lane / 16);
     auto mfma_tile = [&](const unsigned short (*as)[BM][KP], const unsigned short (*bs)[BN][KP], auto&& mid) {
         bf16x8 xh[FN], xl[FN], yh[2], yl[2];
-#pragma unroll
-        for (int j = 0; j < FN; ++j) {
+        auto read_b = [&](int j) {
             const int nn = wn * WTN + j * 16 + lane % 16;
             xh[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&bs[0][nn][swz(nn, kq)]));
             xl[j] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&bs[1][nn][swz(nn, kq)]));
-        }
+        };
         auto read_a = [&](int i, int slot) {
             const int mm = wm * WTM + i * 16 + lane % 16;
             yh[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&as[0][mm][swz(mm, kq)]));
             yl[slot] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const i16x8*>(&as[1][mm][swz(mm, kq)]));
         };
+        // the first row of blocks reads each B fragment pair just ahead of its
+        // MFMAs, so a k-tile starts after four reads rather than 2*FN + 2 (all
+        // waves read at once after the barrier; measured +1-2% on 13824^3)
+        read_b(0);
         read_a(0, 0);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < FM; ++i) {
-            if (i + 1 < FM) read_a(i + 1, (i + 1) & 1);
+            if (i + 1 < FM && i > 0) read_a(i + 1, (i + 1) & 1);
             __builtin_amdgcn_sched_barrier(0);   // keep those reads ahead of row i's MFMAs
             if (i == FM / 2) mid();
+            if (i == 0) {
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    if (j + 1 < FN) read_b(j + 1);
+                    if (j + 1 == FN && FM > 1) read_a(1, 1);
+                    __builtin_amdgcn_sched_barrier(0);
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[0], acc[0][j], 0, 0, 0);
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yl[0], acc[0][j], 0, 0, 0);
+                    acc[0][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[0], acc[0][j], 0, 0, 0);
+                }
+                continue;
+            }
 #pragma unroll
             for (int j = 0; j < FN; ++j) {
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xl[j], yh[i & 1], acc[i][j], 0, 0, 0);
@@ -951,27 +967,15 @@ struct GemmBatcher {
     };
     std::vector<Item*> pending;           // in submission order
     std::vector<hipEvent_t> spare;
-    // pacing: completion events of grouped launches that may still run
-    // (oldest first).  While kPaceDepth of them run, a new GEMM waits for
-    // same-key partners (up to the window) instead of launching alone: the
-    // device has queued work, so the wait costs no device time and lets the
-    // other slots' GEMMs join the next launch.
-    std::deque<hipEvent_t> inflight;
-    std::vector<hipEvent_t> pace_spare;
-    int64_t launches = 0, problems = 0;
+    // per key: moving average of whether a wait for a partner found one; a
+    // key whose GEMMs rarely meet another slot's stops waiting (launches solo)
+    std::map<std::tuple<int, int, int, int, int>, double> met;
+    int64_t launches = 0, problems = 0, solo = 0;
     int max_group = 0;
 };
 
-constexpr int kPaceDepth = 2;
-
-// Grouped launches still running (mu held); retires the finished ones.
-static int batcher_busy(GemmBatcher* b) {
-    while (!b->inflight.empty() && hipEventQuery(b->inflight.front()) == hipSuccess) {
-        b->pace_spare.push_back(b->inflight.front());
-        b->inflight.pop_front();
-    }
-    (void)hipGetLastError();
-    return int(b->inflight.size());
+static std::tuple<int, int, int, int, int> key_tuple(const GroupKey& k) {
+    return std::make_tuple(k.fmt, k.ta, k.tb, k.bm, k.bn);
 }
 
 // Launch up to kMaxGroup pending items of `key` as one grouped kernel (mu held).
@@ -1008,15 +1012,6 @@ static void launch_group(GemmBatcher* b, const GroupKey& key) {
         launch_grouped(key, G, blocks, b->stream, b->cus);
         AQ_HIP(hipGetLastError());
         AQ_HIP(hipEventRecord(L->done, b->stream));
-        hipEvent_t pace = nullptr;
-        if (!b->pace_spare.empty()) {
-            pace = b->pace_spare.back();
-            b->pace_spare.pop_back();
-        } else {
-            AQ_HIP(hipEventCreateWithFlags(&pace, hipEventDisableTiming));
-        }
-        AQ_HIP(hipEventRecord(pace, b->stream));
-        b->inflight.push_back(pace);
     } catch (...) {
         if (L->done) b->spare.push_back(L->done);
         delete L;
@@ -1032,9 +1027,14 @@ static void launch_group(GemmBatcher* b, const GroupKey& key) {
     b->cv.notify_all();
 }
 
-// Hand one GEMM (arguments final, split-K partials allocated) to the batcher:
-// returns once it is launched, with ctx->stream ordered after the grouped kernel.
-static void batcher_submit(aq_ctx* ctx, const GroupKey& key, const GemmArgs& a, int64_t blocks) {
+// Offer one GEMM (arguments final, split-K partials allocated) for a grouped
+// launch.  It joins same-key GEMMs that other slots have pending; when none
+// is pending and another slot is inside a contraction stage, it waits up to
+// the window for one (only for keys whose waits have usually found a
+// partner).  Returns true once it ran in a grouped launch (ctx->stream is
+// then ordered after it), false when it found no partner: the caller launches
+// it alone on its own stream, exactly as without grouping.
+static bool batcher_submit(aq_ctx* ctx, const GroupKey& key, const GemmArgs& a, int64_t blocks) {
     GemmBatcher* b = ctx->batcher;
     if (!ctx->group_ready) AQ_HIP(hipEventCreateWithFlags(&ctx->group_ready, hipEventDisableTiming));
     AQ_HIP(hipEventRecord(ctx->group_ready, ctx->stream));
@@ -1049,18 +1049,27 @@ static void batcher_submit(aq_ctx* ctx, const GroupKey& key, const GemmArgs& a, 
             if (pos != b->pending.end()) b->pending.erase(pos);
         }
     } unlist{b, &item};
-    const auto deadline = std::chrono::steady_clock::now() + b->window;
-    while (!item.launch && !item.failed) {
-        int same = 0;
-        for (const auto* x : b->pending) same += x->key == key ? 1 : 0;
-        const int want = std::min(std::max(b->in_stage, 1), kMaxGroup);
-        const auto now = std::chrono::steady_clock::now();
-        if (same >= want || now >= deadline || batcher_busy(b) < kPaceDepth) {
-            launch_group(b, key);
-            continue;
+    auto same = [&]() {
+        int n = 0;
+        for (const auto* x : b->pending) n += x->key == key ? 1 : 0;
+        return n;
+    };
+    if (same() >= 2) {
+        launch_group(b, key);
+    } else {
+        double& met = b->met.emplace(key_tuple(key), 1.0).first->second;
+        if (b->in_stage >= 2 && met >= 0.2) {
+            const auto deadline = std::chrono::steady_clock::now() + b->window;
+            while (!item.launch && !item.failed && same() < 2 && std::chrono::steady_clock::now() < deadline)
+                b->cv.wait_until(lk, deadline);
+            const bool found = item.launch || same() >= 2;
+            met = 0.9 * met + (found ? 0.1 : 0.0);
+            if (!item.launch && !item.failed && same() >= 2) launch_group(b, key);
         }
-        // poll the device at a short interval (no completion callback)
-        b->cv.wait_until(lk, std::min(deadline, now + std::chrono::microseconds(20)));
+    }
+    if (!item.launch && !item.failed) {   // no partner: the caller launches it alone
+        b->solo += 1;
+        return false;
     }
     AQ_CHECK(!item.failed, AQ_ERR_HIP, "grouped GEMM launch failed");
     GemmBatcher::Launch* L = item.launch;
@@ -1070,6 +1079,7 @@ static void batcher_submit(aq_ctx* ctx, const GroupKey& key, const GemmArgs& a, 
         delete L;
     }
     AQ_HIP(e);
+    return true;
 }
 
 GemmBatcher* batcher_create(int device, int window_us) {
@@ -1088,8 +1098,6 @@ void batcher_destroy(GemmBatcher* b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     for (auto e : b->spare) (void)hipEventDestroy(e);
-    for (auto e : b->inflight) (void)hipEventDestroy(e);
-    for (auto e : b->pace_spare) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(b->stream);
     delete b;
 }
@@ -1100,11 +1108,12 @@ void batcher_stage(GemmBatcher* b, int delta) {
     b->cv.notify_all();
 }
 
-void batcher_stats(GemmBatcher* b, int64_t* launches, int64_t* problems, int* max_group) {
+void batcher_stats(GemmBatcher* b, int64_t* launches, int64_t* problems, int* max_group, int64_t* solo) {
     std::lock_guard<std::mutex> lk(b->mu);
     if (launches) *launches = b->launches;
     if (problems) *problems = b->problems;
     if (max_group) *max_group = b->max_group;
+    if (solo) *solo = b->solo;
 }
 
 void gemm(aq_ctx* ctx, const GemmDesc& d) {
@@ -1174,11 +1183,13 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
             part.reset(new Scratch(ctx, size_t(nsplit) * d.batch * d.M * d.N * 8));
             a.P = part->as<double>();
         }
-        batcher_submit(ctx, GroupKey{int(d.fmt), d.transA ? 1 : 0, d.transB ? 1 : 0, BM, BN}, a,
-                       base_tiles * d.batch * nsplit);
-        if (nsplit > 1)
-            launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
-        return;
+        if (batcher_submit(ctx, GroupKey{int(d.fmt), d.transA ? 1 : 0, d.transB ? 1 : 0, BM, BN}, a,
+                           base_tiles * d.batch * nsplit)) {
+            if (nsplit > 1)
+                launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
+            return;
+        }
+        a.P = nullptr;   // alone: the launch below allocates its own partials
     }
 
     int variant = ctx->gemm_variant[d.fmt];

@@ -285,7 +285,7 @@ void gemm(aq_ctx* ctx, const GemmDesc& d);
 GemmBatcher* batcher_create(int device, int window_us);
 void batcher_destroy(GemmBatcher* b);
 void batcher_stage(GemmBatcher* b, int delta);   // a context enters (+1) or leaves (-1) a stage
-void batcher_stats(GemmBatcher* b, int64_t* launches, int64_t* problems, int* max_group);
+void batcher_stats(GemmBatcher* b, int64_t* launches, int64_t* problems, int* max_group, int64_t* solo);
 
 // Marks the calling context as inside a contraction stage for its batcher.
 struct StageMark {

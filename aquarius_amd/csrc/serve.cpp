@@ -385,16 +385,17 @@ int aq_server_set_grouping(aq_server* s, int window_us) {
     });
 }
 
-int aq_server_grouping_stats(aq_server* s, int64_t* launches, int64_t* gemms, int* max_group) {
+int aq_server_grouping_stats(aq_server* s, int64_t* launches, int64_t* gemms, int* max_group, int64_t* solo) {
     return guarded([&] {
         AQ_CHECK(s, AQ_ERR_INVALID, "null server");
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->batcher) {
-            batcher_stats(s->batcher, launches, gemms, max_group);
+            batcher_stats(s->batcher, launches, gemms, max_group, solo);
         } else {
             if (launches) *launches = 0;
             if (gemms) *gemms = 0;
             if (max_group) *max_group = 0;
+            if (solo) *solo = 0;
         }
     });
 }

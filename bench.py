@@ -41,10 +41,11 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 # of `bench.py --only fp64 --steps 1 --warmup 1 --autotune 0`, scaled by the same counters on a plain 1 GiB copy
 # (fetch x1.9999, write x1.0); mean over the class's launches in the timed iteration.  Keyed by
 # (timing class, o, v); the per-launch tables and the commands are in the profile summaries
-# gemm_f64_128x128_NT_o48v288_hbm_traffic.txt and gemm_bf16x3_256x256_o48v288_hbm_traffic.txt
-# (the BF16x3 class from `bench.py --only mixed --steps 1 --warmup 1`, the timed iteration).
+# gemm_f64_128x128_NT_o48v288_hbm_traffic.txt and bf16x3_iteration_o48v288.txt
+# (the BF16x3 class from `bench.py --only mixed --steps 1 --warmup 1`, the timed iteration, summarised
+# in bf16x3_iteration_o48v288.txt).
 TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 87.7e9,
-                      ("gemm_bf16x3_256x256_TN", 48, 288): 20.2e9}
+                      ("gemm_bf16x3_256x256_TN", 48, 288): 16.79e9}
 
 
 def flops_per_iter(o, v):
@@ -216,41 +217,46 @@ def serve_stream(a, seed):
            "first_error": next((r["error"] for r in res if r["status"] == aq.JOB_FAILED), ""),
            "wall_s": wall, "time_to_E_s": pct(tte), "service_time_s": pct(svc)}
     if a.fp64_sample:
-        # every k-th job, solved one at a time (no arrivals: back to back) in
-        # FP64 and, for the same jobs, in the mixed policy.  Per-job times
-        # vary with the job's size, so the whole stream's one-at-a-time time
-        # is also estimated from the sample scaled by the stream's FLOPs
-        # (executed FLOPs per iteration, summed over jobs).
-        sample_idx = list(range(0, len(jobs), max(1, len(jobs) // a.fp64_sample)))[:a.fp64_sample]
+        # Baselines on identical jobs: every k-th job of the stream (a fixed
+        # subset) is (1) served again as a burst with the same server
+        # settings, (2) solved one at a time in FP64, back to back, and, with
+        # --mixed-one-at-a-time, (3) one at a time in the mixed policy.  The
+        # served/FP64 ratio compares the two on the same work; per-job |dE|
+        # compares each subset job's served energy with its FP64 solve.
+        every = max(1, len(jobs) // a.fp64_sample)
+        sub = [jobs[k] for k in range(0, len(jobs), every)][:a.fp64_sample]
+        idx = [k for k in range(0, len(jobs), every)][:a.fp64_sample]
         c64 = aq.default_config(packed_ladder=a.packed_ladder, convergence=a.convergence, diis_order=a.diis,
                                 max_iterations=150)
-        times, times_mx, dE = [], [], []
-        with aq.Server(slots=1, hbm_budget=int(a.budget_gb * 2**30)) as srv:
-            for k in sample_idx:
-                jseed, o, v, _ = jobs[k]
-                r = srv.wait(srv.submit(jseed, o, v, config=c64))
-                times.append(r["t_end"] - r["t_start"])
-                if res[k]["status"] == aq.JOB_DONE:
-                    dE.append({"o": o, "v": v, "dE": res[k]["energy"] - r["energy"]})
-                rm = srv.wait(srv.submit(jseed, o, v, config=cfg))
-                times_mx.append(rm["t_end"] - rm["t_start"])
-        f_all = sum(flops_per_iter_executed(o, v, a.packed_ladder) for _, o, v, _ in jobs)
-        f_smp = sum(flops_per_iter_executed(jobs[k][1], jobs[k][2], a.packed_ladder) for k in sample_idx)
-        scale = f_all / f_smp
+        with aq.Server(slots=a.slots, hbm_budget=int(a.budget_gb * 2**30), max_bypass_s=a.max_bypass) as srv:
+            if a.grouping_us > 0:
+                srv.set_grouping(a.grouping_us)
+            ids = [srv.submit(js, o, v, config=cfg) for js, o, v, _ in sub]
+            rs = [srv.wait(i) for i in ids]
+            sub_wall = max(r["t_end"] for r in rs) - min(r["t_submit"] for r in rs)
 
-        def rate(ts):
-            return {"mean_solve_s": float(np.mean(ts)), "jobs_per_h_sample_mean": 3600.0 / float(np.mean(ts)),
-                    "jobs_per_h_stream_estimate": 3600.0 * len(jobs) / (float(np.sum(ts)) * scale)}
+        def one_at_a_time(config):
+            ts, es = [], []
+            with aq.Server(slots=1, hbm_budget=int(a.budget_gb * 2**30)) as srv:
+                for js, o, v, _ in sub:
+                    r = srv.wait(srv.submit(js, o, v, config=config))
+                    ts.append(r["t_end"] - r["t_start"])
+                    es.append(r["energy"])
+            return ts, es
 
-        out["one_at_a_time"] = {
-            "sample_jobs": len(sample_idx), "every": max(1, len(jobs) // a.fp64_sample),
-            "stream_flops_over_sample_flops": scale,
-            "fp64": rate(times), "mixed": rate(times_mx),
-            "how": "jobs_per_h_stream_estimate = 3600 * jobs / (sum of the sample's solve times * stream FLOPs / "
-                   "sample FLOPs)"}
-        out["vs_fp64_one_at_a_time"] = out["value"] / out["one_at_a_time"]["fp64"]["jobs_per_h_stream_estimate"]
-        out["vs_mixed_one_at_a_time"] = out["value"] / out["one_at_a_time"]["mixed"]["jobs_per_h_stream_estimate"]
-        out["dE_mixed_vs_fp64_per_sampled_job"] = dE
+        t64, e64 = one_at_a_time(c64)
+        out["subset"] = {"jobs": len(sub), "every": every,
+                         "served_burst_jobs_per_h": 3600.0 * len(sub) / sub_wall,
+                         "fp64_one_at_a_time_jobs_per_h": 3600.0 * len(sub) / float(np.sum(t64)),
+                         "fp64_one_at_a_time_mean_solve_s": float(np.mean(t64))}
+        if a.mixed_one_at_a_time:
+            tmx, _ = one_at_a_time(cfg)
+            out["subset"]["mixed_one_at_a_time_jobs_per_h"] = 3600.0 * len(sub) / float(np.sum(tmx))
+        out["vs_fp64_one_at_a_time"] = (out["subset"]["served_burst_jobs_per_h"] /
+                                        out["subset"]["fp64_one_at_a_time_jobs_per_h"])
+        dE = [{"o": o, "v": v, "dE": res[k]["energy"] - e}
+              for k, (_, o, v, _), e in zip(idx, sub, e64) if res[k]["status"] == aq.JOB_DONE]
+        out["dE_mixed_vs_fp64_per_subset_job"] = dE
         out["max_abs_dE_mixed_vs_fp64"] = max((abs(d["dE"]) for d in dE), default=None)
     return out
 
@@ -471,6 +477,8 @@ def main():
     ap.add_argument("--max-bypass", type=float, default=30.0)
     ap.add_argument("--fp64-sample", type=int, default=32)
     ap.add_argument("--burst", action="store_true", help="serving: submit every job at t = 0 (no arrival gaps)")
+    ap.add_argument("--mixed-one-at-a-time", type=int, default=0,
+                    help="serving: also solve the baseline subset one at a time in the mixed policy")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=0,
                     help="kept for the harness; the timed region is always the complete mixed solve (every "

@@ -553,16 +553,22 @@ int aq_server_drain(aq_server* s);
 int aq_server_stats_get(aq_server* s, aq_server_stats* out);
 /* Cross-job GEMM grouping (the serving loop's grouped launch; no reference
  * counterpart).  window_us > 0: the slots' eager GEMMs with 128x128 or 64x64
- * FP64/BF16x3 blocks are merged, across jobs, into one grouped kernel launch
- * per (format, operand layout, block) key: a GEMM waits on the host until
- * every slot inside a contraction stage has one of the same key pending, or
- * window_us has passed.  Each element is computed exactly as in a launch of
- * its GEMM alone, so results do not change.  Stages replayed from a HIP graph
- * do not group (run jobs with cfg.graph = 0 to group every stage).
- * window_us = 0 turns grouping off.  Only while no job is queued or running. */
+ * FP64/BF16x3 blocks are merged, across jobs, into one persistent grouped
+ * kernel launch per (format, operand layout, block) key: a fixed grid of
+ * resident blocks walks the tiles of every member GEMM.  A GEMM joins
+ * same-key GEMMs that other slots have pending; with none pending, and
+ * another slot inside a contraction stage, it waits up to window_us for one,
+ * but only for keys whose earlier waits usually found a partner (a moving
+ * average above 0.2).  A GEMM that finds no partner launches alone on its own
+ * slot's stream, as without grouping.  Each element is computed exactly as in
+ * a launch of its GEMM alone, so results do not change.  Stages replayed from
+ * a HIP graph do not group (run jobs with cfg.graph = 0 to group every
+ * stage).  window_us = 0 turns grouping off.  Only while no job is queued or
+ * running. */
 int aq_server_set_grouping(aq_server* s, int window_us);
-/* Grouped launches so far, the GEMMs they carried and the largest group. */
-int aq_server_grouping_stats(aq_server* s, int64_t* launches, int64_t* gemms, int* max_group);
+/* Grouped launches so far, the GEMMs they carried, the largest group, and the
+ * GEMMs offered for grouping that found no partner and ran alone. */
+int aq_server_grouping_stats(aq_server* s, int64_t* launches, int64_t* gemms, int* max_group, int64_t* solo);
 /* Drains, stops the workers and releases every context. */
 int aq_server_destroy(aq_server* s);
 

@@ -68,7 +68,7 @@ def test_grouped_launches_equal_alone(prec):
             s.set_grouping(0)          # refused while a job is queued or running
         assert s.wait(extra)["status"] == aq.JOB_DONE
         s.set_grouping(0)
-        assert s.grouping_stats() == {"launches": 0, "gemms": 0, "max_group": 0}
+        assert s.grouping_stats() == {"launches": 0, "gemms": 0, "max_group": 0, "solo": 0}
     print(f"grouping {prec}: {st}")
     assert st["launches"] > 0 and st["gemms"] > st["launches"] and st["max_group"] >= 2
     for a, m, (seed, o, v) in zip(alone, mix, JOBS):
