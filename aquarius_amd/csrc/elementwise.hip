@@ -128,12 +128,11 @@ __global__ __launch_bounds__(TPB) void k_strided(PlanDev p, const double* __rest
 }
 
 // LDS-tiled transpose: dims d0 (unit stride in dst) and d1 (unit stride in
-// src) are tiled T x T; the other dims are walked per block.  Reads are
-// coalesced along d1, writes along d0.  T = 32 for FP64 output (256-byte
-// rows); T = 64 for the BF16 hi/lo planes, so that each plane's row is 128
-// bytes, a whole cache line, instead of 64.
+// src) are tiled T x T (T = 32); the other dims are walked per block.  Reads
+// are coalesced along d1, writes along d0.  (A 64 x 64 tile for the BF16
+// hi/lo output, 128-byte plane rows instead of 64, measured 7% slower on the
+// o=48, v=288 splits: a quarter of the blocks per tile count, twice the LDS.)
 template <class Out> struct TransposeTile { static constexpr int T = 32; };
-template <> struct TransposeTile<BF16X2Out> { static constexpr int T = 64; };
 
 template <class Out>
 __global__ __launch_bounds__(TPB) void k_transpose(PlanDev p, int d0, int d1, int64_t tiles0, int64_t tiles1,

@@ -1158,6 +1158,25 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
         }
         nsplit = int(best);
         while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
+    } else if (d.fmt == FMT_F64 && T > W && T <= 8 * W && ksteps >= 64) {
+        // A grid of one to eight rounds with a ragged last round (the packed
+        // ladder: 3260 tiles = 6.4 rounds): split s ways when that cuts the
+        // rounds per unit of work, ceil(T*s / W) / s, by at least 5%, with at
+        // least 8 k-steps per split and at most 4 GiB of FP64 partials.
+        // FP64 only: the BF16x3 phase keeps its operand copies in that memory.
+        const double r1 = double((T + W - 1) / W);
+        int64_t best = 1;
+        double best_cost = r1;
+        for (int64_t sc = 2; sc <= 8 && ksteps / sc >= 8; ++sc) {
+            if (double(sc) * double(d.batch) * double(d.M) * double(d.N) * 8.0 > 4.0 * double(int64_t(1) << 30)) break;
+            const double cost = double((T * sc + W - 1) / W) / double(sc);
+            if (cost < best_cost - 1e-12) {
+                best_cost = cost;
+                best = sc;
+            }
+        }
+        if (best_cost <= 0.95 * r1) nsplit = int(best);
+        while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
     }
     int64_t kchunk = ((ksteps + nsplit - 1) / nsplit) * KSTEP;
     nsplit = int((d.K + kchunk - 1) / kchunk);

@@ -168,6 +168,8 @@ def serve_stream(a, seed):
     import numpy as np
     import aquarius_amd as aq
     jobs = job_stream(seed, a.jobs, a.o_min, a.o_max, a.rate)
+    if a.serve_every > 1:   # a fixed subset of the stream (every k-th job)
+        jobs = jobs[::a.serve_every]
     if a.burst:   # every job submitted at t = 0: the loop's throughput without arrival gaps
         jobs = [(s_, o_, v_, 0.0) for s_, o_, v_, _ in jobs]
     cfg = aq.default_config(packed_ladder=a.packed_ladder, graph=a.graph, convergence=a.convergence,
@@ -475,8 +477,10 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=240.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--max-bypass", type=float, default=30.0)
-    ap.add_argument("--fp64-sample", type=int, default=32)
+    ap.add_argument("--fp64-sample", type=int, default=16,
+                    help="serving: size of the job subset (every k-th job) for the one-at-a-time baselines")
     ap.add_argument("--burst", action="store_true", help="serving: submit every job at t = 0 (no arrival gaps)")
+    ap.add_argument("--serve-every", type=int, default=1, help="serving: serve only every k-th job of the stream")
     ap.add_argument("--mixed-one-at-a-time", type=int, default=0,
                     help="serving: also solve the baseline subset one at a time in the mixed policy")
     ap.add_argument("--gpus", type=int, default=1)
