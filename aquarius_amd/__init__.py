@@ -74,6 +74,7 @@ _SIGS = {
     "aq_tensor_size": [_P, _I64P],
     "aq_tensor_data": [_P, C.POINTER(_P)],
     "aq_tensor_touch": [_P],
+    "aq_tensor_resize": [_P, C.c_int, C.POINTER(C.c_int64), C.c_int],
     "aq_contract": [_P, C.c_double, _P, C.c_char_p, _P, C.c_char_p, C.c_double, _P, C.c_char_p],
     "aq_sum": [_P, C.c_double, _P, C.c_char_p, C.c_double, _P, C.c_char_p],
     "aq_scale": [_P, _P, C.c_double, C.c_char_p],
@@ -280,6 +281,13 @@ class Tensor:
     def touch(self):
         """Declare an outside write through data_ptr() (aq_tensor_touch)."""
         _ck(lib().aq_tensor_touch(self.h))
+
+    def resize(self, shape, zero=True):
+        """New lengths and fresh storage (aq_tensor_resize; CTFTensor::resize)."""
+        shape = tuple(int(x) for x in shape)
+        ln = (C.c_int64 * max(len(shape), 1))(*shape)
+        _ck(lib().aq_tensor_resize(self.h, len(shape), ln, 1 if zero else 0))
+        self.shape = shape
 
     def write_all(self, arr):
         arr = np.asarray(arr, dtype=np.float64)

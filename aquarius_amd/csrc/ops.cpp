@@ -635,6 +635,43 @@ int aq_tensor_destroy(aq_tensor* t) {
     });
 }
 
+int aq_tensor_resize(aq_tensor* t, int ndim, const int64_t* len, int zero) {
+    return guarded([&] {
+        AQ_CHECK(t, AQ_ERR_INVALID, "null argument");
+        AQ_CHECK(ndim >= 0 && ndim <= 16, AQ_ERR_INVALID, "ndim out of range");
+        AQ_CHECK(ndim == 0 || len, AQ_ERR_INVALID, "null lengths");
+        std::vector<int64_t> l;
+        int64_t size = 1;
+        for (int k = 0; k < ndim; ++k) {
+            AQ_CHECK(len[k] >= 0, AQ_ERR_INVALID, "negative length");
+            l.push_back(len[k]);
+            size *= len[k];
+        }
+        aq_ctx* ctx = t->ctx;
+        // drop cached copies of the old contents, then replace the storage
+        const std::string pre = std::to_string(t->id) + "|";
+        for (auto it = ctx->cache.begin(); it != ctx->cache.end();) {
+            if (it->first.compare(0, pre.size(), pre) == 0 && it->second.pins == 0) {
+                dfree(ctx, it->second.ptr);
+                ctx->cache_bytes -= it->second.bytes;
+                ctx->cache_lru.erase(it->second.lru);
+                it = ctx->cache.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        double* fresh = tensor_alloc(ctx, size_t(size) * 8);
+        tensor_free(ctx, t->data, size_t(t->size) * 8);
+        t->data = fresh;
+        t->ndim = ndim;
+        t->len = l;
+        t->size = size;
+        t->id = new_tensor_id();   // a new tensor as far as any cache is concerned
+        t->version = 0;
+        if (zero) AQ_HIP(hipMemsetAsync(t->data, 0, size_t(std::max<int64_t>(size, 1)) * 8, ctx->stream));
+    });
+}
+
 int aq_tensor_ndim(const aq_tensor* t, int* ndim) {
     return guarded([&] {
         AQ_CHECK(t && ndim, AQ_ERR_INVALID, "null argument");

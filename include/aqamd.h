@@ -130,6 +130,12 @@ int aq_tensor_create(aq_ctx* ctx, int ndim, const int64_t* len, int zero, aq_ten
 int aq_tensor_copy(aq_ctx* ctx, const aq_tensor* src, aq_tensor** out);
 /* ~CTFTensor (ctf_tensor.cxx:123-127). */
 int aq_tensor_destroy(aq_tensor* t);
+/* CTFTensor::resize(ndim, len, sym, zero) (ctf_tensor.cxx:189-201): new
+ * lengths (all NS), the old data freed and new storage allocated, cleared
+ * when zero != 0 (otherwise undefined).  The handle stays valid; copies the
+ * library cached from the old contents are dropped.  Not for a tensor that a
+ * live solver borrows as an input. */
+int aq_tensor_resize(aq_tensor* t, int ndim, const int64_t* len, int zero);
 int aq_tensor_ndim(const aq_tensor* t, int* ndim);
 int aq_tensor_lengths(const aq_tensor* t, int64_t* len);
 int aq_tensor_size(const aq_tensor* t, int64_t* size);

@@ -338,3 +338,26 @@ def test_invalid_arguments_raise(ctx):
         ctx.norm(t, 3)
     t.free()
     u.free()
+
+
+def test_resize_gives_fresh_storage(ctx):
+    """CTFTensor::resize (ctf_tensor.cxx:189-201): new lengths, the old
+    contents gone (cleared when asked), no cached copy of the old contents
+    reused by a later contraction."""
+    rng = np.random.default_rng(31)
+    A = ctx.from_numpy(rng.standard_normal((12, 7)))
+    B = ctx.from_numpy(rng.standard_normal((12, 5)))
+    C1 = ctx.tensor((7, 5))
+    ctx.contract(1.0, A, "km", B, "kn", 0.0, C1, "mn")     # caches a copy of A
+    A.resize((12, 9), zero=True)
+    assert A.numpy().shape == (12, 9) and not A.numpy().any()
+    a2 = rng.standard_normal((12, 9))
+    A.write_all(a2)
+    C2 = ctx.tensor((9, 5))
+    ctx.contract(1.0, A, "km", B, "kn", 0.0, C2, "mn")
+    ref = a2.T @ B.numpy()
+    np.testing.assert_allclose(C2.numpy(), ref, rtol=1e-12, atol=1e-12)
+    A.resize((3,), zero=True)
+    assert A.numpy().tolist() == [0.0, 0.0, 0.0]
+    for t in (A, B, C1, C2):
+        t.free()
