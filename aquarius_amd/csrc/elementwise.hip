@@ -639,20 +639,60 @@ __global__ __launch_bounds__(TPB) void k_ladder_pack_v(const double* __restrict_
     }
 }
 
-// T+ [P x O+] and T- [Q x O-] from Tau[v,v,o,o]; grid.y walks the (i <= j) columns.
+// T+ [O+ x P] and T- [O- x Q] from Tau[v,v,o,o]: the occupied pair index is
+// the contiguous one, so the ladder GEMM S = V T^T reads both operands
+// row-contiguous.  Each block packs a tile of 64 (e <= f) pairs x 32 (i <= j)
+// pairs: Tau is read along e (its contiguous index) and the tile is written
+// through LDS along the pair (i,j).
+constexpr int PK_R = 64, PK_C = 32;
 __global__ __launch_bounds__(TPB) void k_ladder_pack_t(const double* __restrict__ T, int64_t v, int64_t o,
                                                        double* __restrict__ Tp, double* __restrict__ Tm) {
-    const int64_t P = v * (v + 1) / 2, Q = v * (v - 1) / 2, Op = o * (o + 1) / 2;
-    for (int64_t c = blockIdx.y; c < Op; c += gridDim.y) {
-        int64_t i, j;
-        untri_le(c, i, j);
-        const double* Tij = T + (i + j * o) * v * v;
-        for (int64_t r = int64_t(blockIdx.x) * TPB + threadIdx.x; r < P; r += int64_t(gridDim.x) * TPB) {
-            int64_t e, f;
-            untri_le(r, e, f);
-            const double x = Tij[e + f * v], y = Tij[f + e * v];
-            Tp[r + c * P] = 0.5 * (x + y);
-            if (e < f && i < j) Tm[tri_lt(e, f) + tri_lt(i, j) * Q] = 0.5 * (x - y);
+    static_assert(TPB == 256 && PK_R * PK_C == 8 * TPB, "tile of 8 elements per thread");
+    __shared__ double sp[PK_R][PK_C + 1], sm[PK_R][PK_C + 1];
+    const int64_t P = v * (v + 1) / 2, Op = o * (o + 1) / 2, Om = o * (o - 1) / 2;
+    const int64_t r0 = int64_t(blockIdx.x) * PK_R, c0 = int64_t(blockIdx.y) * PK_C;
+    const int tid = threadIdx.x;
+    {
+        // read: thread -> one (e,f) row, eight (i,j) columns
+        const int rl = tid % PK_R;
+        const int64_t r = r0 + rl;
+        int64_t e = 0, f = 0;
+        if (r < P) untri_le(r, e, f);
+#pragma unroll
+        for (int s = 0; s < PK_C / (TPB / PK_R); ++s) {
+            const int cl = tid / PK_R + s * (TPB / PK_R);
+            const int64_t c = c0 + cl;
+            double x = 0.0, y = 0.0;
+            if (r < P && c < Op) {
+                int64_t i, j;
+                untri_le(c, i, j);
+                const double* Tij = T + (i + j * o) * v * v;
+                x = Tij[e + f * v];
+                y = Tij[f + e * v];
+            }
+            sp[rl][cl] = 0.5 * (x + y);
+            sm[rl][cl] = 0.5 * (x - y);
+        }
+    }
+    __syncthreads();
+    {
+        // write: thread -> one (i,j) column, eight (e,f) rows
+        const int cl = tid % PK_C;
+        const int64_t c = c0 + cl;
+        int64_t i = 0, j = 0;
+        if (c < Op) untri_le(c, i, j);
+#pragma unroll
+        for (int s = 0; s < PK_R / (TPB / PK_C); ++s) {
+            const int rl = tid / PK_C + s * (TPB / PK_C);
+            const int64_t r = r0 + rl;
+            if (r >= P || c >= Op) continue;
+            Tp[c + r * Op] = sp[rl][cl];
+            if (i < j) {
+                int64_t e, f;
+                untri_le(r, e, f);
+                // tri_lt(i,j) = tri_le(i,j) - j, tri_lt(e,f) = tri_le(e,f) - f
+                if (e < f) Tm[(c - j) + (r - f) * Om] = sm[rl][cl];
+            }
         }
     }
 }
@@ -708,9 +748,10 @@ void launch_ladder_pack_v(aq_ctx* ctx, const double* V, int v, double* Vp, doubl
 void launch_ladder_pack_t(aq_ctx* ctx, const double* T, int v, int o, double* Tp, double* Tm) {
     const int64_t P = int64_t(v) * (v + 1) / 2, Op = int64_t(o) * (o + 1) / 2;
     TimedLaunch tl(ctx, "ladder_pack_t", 0.0, 8.0 * (2.0 * P * Op + 2.0 * P * Op));
-    const unsigned gx = unsigned(std::min<int64_t>((P + TPB - 1) / TPB, 64));
-    const unsigned gy = unsigned(std::min<int64_t>(Op, 65535));
-    hipLaunchKernelGGL(k_ladder_pack_t, dim3(gx, gy), dim3(TPB), 0, ctx->stream, T, int64_t(v), int64_t(o), Tp, Tm);
+    const int64_t gx = (P + PK_R - 1) / PK_R, gy = (Op + PK_C - 1) / PK_C;
+    AQ_CHECK(gx < (int64_t(1) << 31) && gy < 65536, AQ_ERR_UNSUPPORTED, "packed ladder too large");
+    hipLaunchKernelGGL(k_ladder_pack_t, dim3(unsigned(gx), unsigned(gy)), dim3(TPB), 0, ctx->stream, T, int64_t(v),
+                       int64_t(o), Tp, Tm);
     AQ_HIP(hipGetLastError());
 }
 

@@ -31,6 +31,7 @@
 #include <cmath>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <map>
 #include <memory>
@@ -83,6 +84,18 @@ __device__ __forceinline__ int64_t remap_tile(int64_t bid, int64_t nb) {
     const int64_t full = (nb / 8) * 8;
     if (bid >= full) return bid;
     return (bid % 8) * (nb / 8) + bid / 8;
+}
+
+// Block (bx, bz) of a launch with grid (tiles, 1, slices) -> (tile, slice).
+// The dispatcher deals blocks to the XCDs round-robin in linear order (x
+// fastest, then z); remapping the linear id over all tiles x slices gives each
+// XCD a contiguous run of tiles of one batch/split slice, whose operand
+// panels it then shares in its L2 (remapping x alone scatters a slice over
+// every XCD when the tile count is not a multiple of 8).
+__device__ __forceinline__ void remap_block(int64_t tiles, int64_t& t, int& bz) {
+    const int64_t r = remap_tile(t + tiles * bz, tiles * int64_t(gridDim.z));
+    t = r % tiles;
+    bz = int(r / tiles);
 }
 
 // Result layout of the 16x16 MFMAs (measured on gfx950): lane l, register r
@@ -183,29 +196,43 @@ struct TileLoader {
     }
 };
 
-// Loader of complete k-tiles of a tile whose rows all lie inside the operand
-// (same element assignment as TileLoader).  Element r of k-tile kt is at
-// org + kt*kstep (the buffer resource, uniform) + r*rstep (scalar offset) +
-// this thread's byte offset boff (vector offset): a load needs no address
-// arithmetic of its own and no mask.  Every offset is below 2^31 bytes
-// (GemmArgs::off32).  The caller owns the staging registers, so several
-// k-tiles can be in flight.
-template <typename T, int ROWS, int BK, bool KC>
+// Loader of complete k-tiles (same element assignment as TileLoader).
+// Element r of k-tile kt is at org + kt*kstep (the buffer resource, uniform)
+// + r*rstep (scalar offset) + this thread's byte offset boff (vector offset):
+// a load needs no address arithmetic of its own and no mask.  Every offset is
+// below 2^31 bytes (GemmArgs::off32).  The caller owns the staging registers,
+// so several k-tiles can be in flight.
+// EDGE: the tile's rows may run past the operand (nrows).  Such rows read the
+// operand's last row instead (every address stays inside the operand); they
+// only feed accumulators of output rows or columns that are never stored, so
+// the values do not matter.  Row-contiguous operands clamp the thread's one
+// row; k-contiguous ones keep one clamped vector offset per element.
+template <typename T, int ROWS, int BK, bool KC, bool EDGE = false>
 struct FastLoader {
     static constexpr int NE = ROWS * BK / 256;
     static constexpr int VSTEP = 256 / (KC ? BK : ROWS);
+    static constexpr int NOFF = (EDGE && KC) ? NE : 1;
     const T* org;
     int64_t kstep, rstep;
-    uint32_t boff;
+    uint32_t boff[NOFF];
     int fixed, var0;
 
-    __device__ __forceinline__ void init(const T* P, int64_t ld, int64_t row0, int64_t kbeg, int tid) {
+    __device__ __forceinline__ void init(const T* P, int64_t ld, int64_t row0, int64_t nrows, int64_t kbeg, int tid) {
         fixed = KC ? tid % BK : tid % ROWS;
         var0 = KC ? tid / BK : tid / ROWS;
         org = KC ? P + row0 * ld + kbeg : P + kbeg * ld + row0;
         kstep = KC ? BK : BK * ld;
         rstep = int64_t(VSTEP) * ld;
-        boff = uint32_t((int64_t(var0) * ld + fixed) * int64_t(sizeof(T)));
+        const int last = int(min<int64_t>(nrows - row0, ROWS)) - 1;   // last row of the tile inside the operand
+        if (!EDGE) {
+            boff[0] = uint32_t((int64_t(var0) * ld + fixed) * int64_t(sizeof(T)));
+        } else if (!KC) {
+            boff[0] = uint32_t((int64_t(var0) * ld + min(fixed, last)) * int64_t(sizeof(T)));
+        } else {
+#pragma unroll
+            for (int r = 0; r < NOFF; ++r)
+                boff[r] = uint32_t((int64_t(min(var0 + r * VSTEP, last)) * ld + fixed) * int64_t(sizeof(T)));
+        }
     }
     __device__ __forceinline__ void load(int64_t kt, T (&v)[NE]) const {
         const __amdgpu_buffer_rsrc_t rs =
@@ -213,11 +240,12 @@ struct FastLoader {
 #pragma unroll
         for (int r = 0; r < NE; ++r) {
             // uniform by construction; readfirstlane keeps it in a scalar register
-            const int so = __builtin_amdgcn_readfirstlane(int(int64_t(r) * rstep * int64_t(sizeof(T))));
+            const int so = (EDGE && KC) ? 0 : __builtin_amdgcn_readfirstlane(int(int64_t(r) * rstep * int64_t(sizeof(T))));
+            const uint32_t vo = boff[(EDGE && KC) ? r : 0];
             if constexpr (sizeof(T) == 8)
-                v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, boff, so, 0));
+                v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rs, vo, so, 0));
             else
-                v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, boff, so, 0));
+                v[r] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rs, vo, so, 0));
         }
     }
     template <int LD>
@@ -250,7 +278,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
 
     const int64_t tilesN = (g.N + BN - 1) / BN;
     const int64_t tilesM = (g.M + BM - 1) / BM;
-    const int64_t t = REMAP ? remap_tile(bx, tilesM * tilesN) : bx;
+    int64_t t = bx;
+    if (REMAP) remap_block(tilesM * tilesN, t, bz);
     int64_t tmi, tni;
     tile_of(t, tilesM, tilesN, g.raster, tmi, tni);
     const int64_t m0 = tmi * BM, n0 = tni * BN;
@@ -264,8 +293,17 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
 #pragma unroll
         for (int j = 0; j < FN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-    // one k-tile of MFMAs from LDS buffer buf
-    auto compute = [&](int buf) {
+    // Wave-uniform counts of this wave's 16-row / 16-column blocks that hold
+    // any output inside C: on an edge tile the MFMAs of the other blocks are
+    // skipped (their accumulators are never stored).
+    const int ilim = __builtin_amdgcn_readfirstlane(
+        int(max<int64_t>(0, min<int64_t>(FM, (g.M - (m0 + wm * WTM) + 15) / 16))));
+    const int jlim = __builtin_amdgcn_readfirstlane(
+        int(max<int64_t>(0, min<int64_t>(FN, (g.N - (n0 + wn * WTN) + 15) / 16))));
+
+    // one k-tile of MFMAs from LDS buffer buf (LIM: skip blocks outside C)
+    auto compute = [&](int buf, auto LIM) {
+        constexpr bool lim = decltype(LIM)::value;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + lane / 16;
@@ -277,7 +315,8 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
 #pragma unroll
             for (int i = 0; i < FM; ++i)
 #pragma unroll
-                for (int j = 0; j < FN; ++j) acc[i][j] = Mfma<T>::run(x[j], y[i], acc[i][j]);
+                for (int j = 0; j < FN; ++j)
+                    if (!lim || (i < ilim && j < jlim)) acc[i][j] = Mfma<T>::run(x[j], y[i], acc[i][j]);
         }
     };
 
@@ -286,15 +325,16 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
     const T* Bb = static_cast<const T*>(g.B) + int64_t(b) * g.strideB;
     const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
     const int64_t nfull = kend > kbeg ? (kend - kbeg) / BK : 0;
-    if (g.off32 && m0 + BM <= g.M && n0 + BN <= g.N) {
-        // Interior tile: complete k-tiles through FastLoader, two k-tiles in
-        // flight (the tile stored now was loaded two steps ago), so a load
-        // has two tiles of MFMA work to arrive.  Register set s and LDS
-        // buffer s serve the k-tiles of parity s.
-        FastLoader<T, BM, BK, TA> fa;
-        FastLoader<T, BN, BK, !TB> fb;
-        fa.init(Ab, g.lda, m0, kbeg, tid);
-        fb.init(Bb, g.ldb, n0, kbeg, tid);
+    // Complete k-tiles through FastLoader, two k-tiles in flight (the tile
+    // stored now was loaded two steps ago), so a load has two tiles of MFMA
+    // work to arrive.  Register set s and LDS buffer s serve the k-tiles of
+    // parity s.  EDGE: the tile runs past M or N (clamped rows, skipped blocks).
+    auto fast = [&](auto EDGE) {
+        constexpr bool edge = decltype(EDGE)::value;
+        FastLoader<T, BM, BK, TA, edge> fa;
+        FastLoader<T, BN, BK, !TB, edge> fb;
+        fa.init(Ab, g.lda, m0, g.M, kbeg, tid);
+        fb.init(Bb, g.ldb, n0, g.N, kbeg, tid);
         T va[2][FastLoader<T, BM, BK, TA>::NE], vb[2][FastLoader<T, BN, BK, !TB>::NE];
         if (nfull > 0) { fa.load(0, va[0]); fb.load(0, vb[0]); }
         if (nfull > 1) { fa.load(1, va[1]); fb.load(1, vb[1]); }
@@ -305,7 +345,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
             __syncthreads();
             if (kt + 2 < nfull) { fa.load(kt + 2, va[s]); fb.load(kt + 2, vb[s]); }
             __builtin_amdgcn_sched_barrier(0);
-            compute(s);
+            compute(s, EDGE);
         };
         for (int64_t kt = 0; kt < nfull; kt += 2) {
             step(std::integral_constant<int, 0>{}, kt);
@@ -322,10 +362,15 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
             la.store(As[buf]);
             lb.store(Bs[buf]);
             __syncthreads();
-            compute(buf);
+            compute(buf, EDGE);
         }
+    };
+    if (g.off32 && m0 + BM <= g.M && n0 + BN <= g.N) {
+        fast(std::false_type{});
+    } else if (g.off32) {
+        fast(std::true_type{});
     } else {
-        // edge tile: predicated loads, one k-tile in flight
+        // offsets beyond 32 bits: predicated loads, one k-tile in flight
         TileLoader<T, BM, BK, TA> la;
         TileLoader<T, BN, BK, !TB> lb;
         la.init(Ab, g.lda, m0, g.M, kbeg, tid);
@@ -340,7 +385,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
             // latency hides behind the matrix work
             if (kt + 1 < nk) { la.load(kt + 1, kbeg, kend); lb.load(kt + 1, kbeg, kend); }
             __builtin_amdgcn_sched_barrier(0);
-            compute(buf);
+            compute(buf, std::true_type{});
         }
     }
 
@@ -429,7 +474,8 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
 
     const int64_t tilesN = (g.N + BN - 1) / BN;
     const int64_t tilesM = (g.M + BM - 1) / BM;
-    const int64_t t = REMAP ? remap_tile(bx, tilesM * tilesN) : bx;
+    int64_t t = bx;
+    if (REMAP) remap_block(tilesM * tilesN, t, bz);
     int64_t tmi, tni;
     tile_of(t, tilesM, tilesN, g.raster, tmi, tni);
     const int64_t m0 = tmi * BM, n0 = tni * BN;
@@ -756,8 +802,9 @@ static void run_cfg(aq_ctx* ctx, const GemmArgs& a, dim3 grid, int variant) {
             hipLaunchKernelGGL((k_gemm<T, BM, BN, WM, WN, false, false, BK_, MW_>), grid, dim3(256), 0, ctx->stream, a); \
     } while (0)
     // variants of the large tile: 0 = BK16 one wave/SIMD, 1 = BK32, 2 = BK16 register-lean (two waves/SIMD)
+    // the one-short-side tiles (128 x 64, 64 x 128) always run two waves per SIMD
     if (BM == 128 && BN == 128 && variant == 1) AQ_LAUNCH(32, 1);
-    else if (BM == 128 && BN == 128 && variant == 2) AQ_LAUNCH(16, 2);
+    else if (BM * BN >= 8192 && variant == 2) AQ_LAUNCH(16, 2);
     else AQ_LAUNCH(16, 1);
 #undef AQ_LAUNCH
 }
@@ -776,11 +823,15 @@ static void run_cfg_bf16(aq_ctx* ctx, const GemmArgs& a, dim3 grid) {
 static void dispatch(aq_ctx* ctx, Fmt fmt, int BM, int BN, int variant, const GemmArgs& a, dim3 grid) {
     if (fmt == FMT_F64) {
         if (BM == 128 && BN == 128) run_cfg<double, 128, 128, 2, 2>(ctx, a, grid, variant);
+        else if (BM == 128 && BN == 64) run_cfg<double, 128, 64, 2, 2>(ctx, a, grid, 2);
+        else if (BM == 64 && BN == 128) run_cfg<double, 64, 128, 2, 2>(ctx, a, grid, 2);
         else if (BM == 64) run_cfg<double, 64, 64, 2, 2>(ctx, a, grid, 0);
         else if (BN == 16) run_cfg<double, 128, 16, 4, 1>(ctx, a, grid, 0);
         else run_cfg<double, 16, 128, 1, 4>(ctx, a, grid, 0);
     } else if (fmt == FMT_F32) {
         if (BM == 128 && BN == 128) run_cfg<float, 128, 128, 2, 2>(ctx, a, grid, variant);
+        else if (BM == 128 && BN == 64) run_cfg<float, 128, 64, 2, 2>(ctx, a, grid, 2);
+        else if (BM == 64 && BN == 128) run_cfg<float, 64, 128, 2, 2>(ctx, a, grid, 2);
         else if (BM == 64) run_cfg<float, 64, 64, 2, 2>(ctx, a, grid, 0);
         else if (BN == 16) run_cfg<float, 128, 16, 4, 1>(ctx, a, grid, 0);
         else run_cfg<float, 16, 128, 1, 4>(ctx, a, grid, 0);
@@ -1129,6 +1180,10 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     if (d.N <= 16) { BM = 128; BN = 16; }
     else if (d.M <= 16) { BM = 16; BN = 128; }
     else if (((d.M + 127) / 128) * ((d.N + 127) / 128) * d.batch < 256) { BM = 64; BN = 64; }
+    // FP64/FP32 with one short side (o = 48 against v^2 o = 4.0e6 rows): a
+    // 64-wide block runs 64/48 of the useful MFMAs where 128 ran 128/48
+    else if (d.fmt != FMT_BF16X2 && d.N <= 64) { BM = 128; BN = 64; }
+    else if (d.fmt != FMT_BF16X2 && d.M <= 64) { BM = 64; BN = 128; }
 
     // Split-K when the output alone cannot fill the device.  The decision and
     // the k-partition depend only on the shape (base tile grid, 32-wide
@@ -1230,6 +1285,15 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     // (N = opA rows / opB cols contiguous, T = k contiguous)
     snprintf(name, sizeof(name), "%s_%dx%d_%c%c", d.tag ? d.tag : "gemm", BM, BN, d.transA ? 'T' : 'N',
              d.transB ? 'T' : 'N');
+    // AQ_GEMM_TRACE=1: one line per launch on stderr (shape, layout, tile, variant, split)
+    static const bool trace = [] {
+        const char* e = getenv("AQ_GEMM_TRACE");
+        return e && e[0] == '1';
+    }();
+    if (trace)
+        fprintf(stderr, "aq_gemm %s M=%lld N=%lld K=%lld batch=%lld beta=%g variant=%d nsplit=%d blocks=%lld\n", name,
+                (long long)d.M, (long long)d.N, (long long)d.K, (long long)d.batch, d.beta, variant, nsplit,
+                (long long)(tiles * d.batch * nsplit));
     {
         TimedLaunch tl(ctx, name, flops, bytes);
         dispatch(ctx, d.fmt, BM, BN, variant, a, grid);

@@ -153,8 +153,8 @@ struct aq_rhfccsd {
         if (!Vp.handle()) {
             Vp = mk({P, P});
             Vm = mk({Q, Q});
-            Tp = mk({P, Op});
-            Tm = mk({Q, Om});
+            Tp = mk({Op, P});   // occupied pair fastest: the ladder GEMM reads T row-contiguous
+            Tm = mk({Om, Q});
             Sp = mk({P, Op});
             Sm = mk({Q, Om});
         }
@@ -274,8 +274,8 @@ struct aq_rhfccsd {
             launch_ladder_pack_t(ctx, dptr(Tau), v, o, dptr(Tp), dptr(Tm));   // as S+ + S- on packed pairs
             bump_t(Tp);
             bump_t(Tm);
-            Sp["pi"] = Vp["pq"] * Tp["qi"];
-            Sm["pi"] = Vm["pq"] * Tm["qi"];
+            Sp["pi"] = Vp["pq"] * Tp["iq"];
+            Sm["pi"] = Vm["pq"] * Tm["iq"];
             launch_ladder_unpack(ctx, dptr(Sp), dptr(Sm), v, o, 0.5, dptr(Z2));
             bump_t(Z2);
         } else {

@@ -646,6 +646,10 @@ def main():
                                                    for vv in (0, 1, 2, 3, 4)}
                                                for p, c in aq.PRECISIONS.items()}}
     ints.free()
+    # release the large system's cached operand copies and pooled scratch here,
+    # outside any timed region, so the small solve below starts as it would in
+    # a context of its own
+    ctx.trim()
 
     # ---- the CPU leg's system on the GPU (FP64 solve to convergence)
     cpu_sys = tuple(int(x) for x in a.cpu_sample.split(",")) if a.cpu_sample else None

@@ -81,6 +81,18 @@ def test_contract_large_tile_path(ctx):
     check_contract(ctx, "ak", "kb", "ba", dim, beta=1.0, seed=3)   # swapped output
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6)])
+@pytest.mark.parametrize("ia,ib", [("mk", "kn"), ("km", "kn"), ("mk", "nk"), ("km", "nk")])
+def test_contract_one_short_side_tiles(ctx, prec, tol, ia, ib):
+    """128 x 64 blocks (N = 48 against 33000 rows) and 64 x 128 blocks (M = 40
+    against 33001 columns): every tile runs past N (or M), so every tile takes
+    the complete-k-tile path with clamped operand rows and skipped MFMA blocks;
+    all four operand layouts, a partial last k-tile (K = 72, 37)."""
+    print(f"case: one short side {prec} {ia},{ib}", flush=True)
+    check_contract(ctx, ia, ib, "mn", {"m": 33000, "n": 48, "k": 72}, seed=31, tol=tol, prec=prec)
+    check_contract(ctx, ia, ib, "mn", {"m": 40, "n": 33001, "k": 37}, seed=32, tol=tol, prec=prec)
+
+
 def test_contract_batch_and_special_labels(ctx):
     dim = {"a": 7, "b": 5, "k": 9, "z": 3, "x": 4}
     check_contract(ctx, "akz", "kbz", "abz", dim, seed=4)          # batch label z

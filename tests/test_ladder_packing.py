@@ -58,6 +58,7 @@ def pack_v(V, v):
 
 
 def pack_t(T, v, o):
+    """T+ [O+ x P] and T- [O- x Q]: the occupied pair index fastest."""
     P, Q, Op, Om = v * (v + 1) // 2, v * (v - 1) // 2, o * (o + 1) // 2, o * (o - 1) // 2
     Tf = T.reshape(-1, order="F")
     Tp, Tm = np.full(P * Op, np.nan), np.full(max(Q * Om, 1), np.nan)
@@ -68,12 +69,14 @@ def pack_t(T, v, o):
             e, f = untri_le(r)
             x, y = Tf[base + e + f * v], Tf[base + f + e * v]
             assert base + max(e + f * v, f + e * v) < v * v * o * o
-            Tp[r + c * P] = 0.5 * (x + y)
+            Tp[c + r * Op] = 0.5 * (x + y)
             if e < f and i < j:
-                k = tri_lt(e, f) + tri_lt(i, j) * Q
+                # the kernel's form of tri_lt(i,j) + tri_lt(e,f) * Om
+                assert c - j == tri_lt(i, j) and r - f == tri_lt(e, f)
+                k = (c - j) + (r - f) * Om
                 assert 0 <= k < Q * Om
                 Tm[k] = 0.5 * (x - y)
-    return Tp.reshape(P, Op, order="F"), Tm[:Q * Om].reshape(Q, Om, order="F")
+    return Tp.reshape(Op, P, order="F"), Tm[:Q * Om].reshape(Om, Q, order="F")
 
 
 def unpack(Sp, Sm, v, o, alpha, Z):
@@ -116,7 +119,7 @@ def test_packed_ladder_equals_full_contraction(v, o):
     assert not np.isnan(Vp).any() and not np.isnan(Vm).any()
     assert not np.isnan(Tp).any() and not np.isnan(Tm).any()
     Z0 = rng.standard_normal((v, v, o, o))
-    Z = unpack(Vp @ Tp, Vm @ Tm, v, o, 0.5, Z0)
+    Z = unpack(Vp @ Tp.T, Vm @ Tm.T, v, o, 0.5, Z0)
     ref = Z0 + 0.5 * np.einsum("abef,efij->abij", V, Tau)
     np.testing.assert_allclose(Z, ref, rtol=1e-12, atol=1e-12)
 
@@ -139,7 +142,7 @@ def test_asymmetric_input_breaks_the_identity():
     _, Tau = symmetric_inputs(rng, v, o)
     Vp, Vm = pack_v(V, v)
     Tp, Tm = pack_t(Tau, v, o)
-    Z = unpack(Vp @ Tp, Vm @ Tm, v, o, 1.0, np.zeros((v, v, o, o)))
+    Z = unpack(Vp @ Tp.T, Vm @ Tm.T, v, o, 1.0, np.zeros((v, v, o, o)))
     ref = np.einsum("abef,efij->abij", V, Tau)
     assert np.abs(Z - ref).max() > 1e-3
 
