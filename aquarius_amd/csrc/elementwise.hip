@@ -697,6 +697,41 @@ __global__ __launch_bounds__(TPB) void k_ladder_pack_t(const double* __restrict_
     }
 }
 
+// Tau, TauSA and T2SA of one iteration in one pass (rhfccsd.cxx:231-238):
+//   Tau(ab,ij)   = T2(ab,ij) + T1(a,i) T1(b,j)
+//   TauSA(ab,ij) = 2 Tau(ab,ij) - Tau(ab,ji)
+//   T2SA(ab,ij)  = 2 T2(ab,ij) - T2(ab,ji)
+// Each block row handles the slab pair (i,j), (j,i) with i <= j, so T2 is
+// read once and both slabs' outputs are written from registers.  Every value
+// is rounded as the operator sequence rounds it (the product, then the sum;
+// 2x exactly, then one rounded difference): contraction into FMAs is off.
+__global__ __launch_bounds__(TPB) void k_tau_sa(const double* __restrict__ T1, const double* __restrict__ T2,
+                                                int64_t v, int64_t o, double* __restrict__ Tau,
+                                                double* __restrict__ TauSA, double* __restrict__ T2SA) {
+#pragma clang fp contract(off)
+    const int64_t vv = v * v, np = o * (o + 1) / 2;
+    for (int64_t c = blockIdx.y; c < np; c += gridDim.y) {
+        int64_t i, j;
+        untri_le(c, i, j);
+        const int64_t sij = (i + j * o) * vv, sji = (j + i * o) * vv;
+        for (int64_t r = int64_t(blockIdx.x) * TPB + threadIdx.x; r < vv; r += int64_t(gridDim.x) * TPB) {
+            const int64_t a = r % v, b = r / v;
+            const double x = T2[sij + r], y = T2[sji + r];
+            const double pij = T1[a + i * v] * T1[b + j * v];
+            const double pji = T1[a + j * v] * T1[b + i * v];
+            const double tij = x + pij, tji = y + pji;
+            Tau[sij + r] = tij;
+            TauSA[sij + r] = 2.0 * tij - tji;
+            T2SA[sij + r] = 2.0 * x - y;
+            if (i != j) {
+                Tau[sji + r] = tji;
+                TauSA[sji + r] = 2.0 * tji - tij;
+                T2SA[sji + r] = 2.0 * y - x;
+            }
+        }
+    }
+}
+
 // Z[v,v,o,o] += alpha * (S+ + S-) expanded to all (a,b,i,j)
 __global__ __launch_bounds__(TPB) void k_ladder_unpack(const double* __restrict__ Sp, const double* __restrict__ Sm,
                                                        int64_t v, int64_t o, double alpha, double* __restrict__ Z) {
@@ -752,6 +787,17 @@ void launch_ladder_pack_t(aq_ctx* ctx, const double* T, int v, int o, double* Tp
     AQ_CHECK(gx < (int64_t(1) << 31) && gy < 65536, AQ_ERR_UNSUPPORTED, "packed ladder too large");
     hipLaunchKernelGGL(k_ladder_pack_t, dim3(unsigned(gx), unsigned(gy)), dim3(TPB), 0, ctx->stream, T, int64_t(v),
                        int64_t(o), Tp, Tm);
+    AQ_HIP(hipGetLastError());
+}
+
+void launch_tau_sa(aq_ctx* ctx, const double* T1, const double* T2, int v, int o, double* Tau, double* TauSA,
+                   double* T2SA) {
+    const int64_t vv = int64_t(v) * v, np = int64_t(o) * (o + 1) / 2;
+    TimedLaunch tl(ctx, "tau_sa", 0.0, 8.0 * 4.0 * vv * o * o);
+    const unsigned gx = unsigned(std::min<int64_t>((vv + TPB - 1) / TPB, 64));
+    const unsigned gy = unsigned(std::min<int64_t>(np, 65535));
+    hipLaunchKernelGGL(k_tau_sa, dim3(gx, gy), dim3(TPB), 0, ctx->stream, T1, T2, int64_t(v), int64_t(o), Tau, TauSA,
+                       T2SA);
     AQ_HIP(hipGetLastError());
 }
 

@@ -251,6 +251,9 @@ void launch_ccsd_update(aq_ctx* ctx, const UpdateArgs& a, int slot_e1, int slot_
 void launch_ladder_pack_v(aq_ctx* ctx, const double* V, int v, double* Vp, double* Vm);
 void launch_ladder_pack_t(aq_ctx* ctx, const double* T, int v, int o, double* Tp, double* Tm);
 void launch_ladder_unpack(aq_ctx* ctx, const double* Sp, const double* Sm, int v, int o, double alpha, double* Z);
+// Tau, TauSA, T2SA from T1 and T2 in one pass (rhfccsd.cxx:231-238)
+void launch_tau_sa(aq_ctx* ctx, const double* T1, const double* T2, int v, int o, double* Tau, double* TauSA,
+                   double* T2SA);
 // max |X(p,q,r,s) - X(q,p,s,r)| of X[n1,n1,n2,n2] (synchronising)
 double pair_asymmetry(aq_ctx* ctx, const double* X, int n1, int n2, int slot);
 

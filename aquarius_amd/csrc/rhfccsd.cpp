@@ -210,12 +210,14 @@ struct aq_rhfccsd {
     // The contraction stage of one iteration (rhfccsd.cxx:231-309): every
     // operation is stream-ordered and none synchronises with the host.
     void contractions() {
-        Tau["abij"] = T2["abij"];                                  // :231
-        Tau["abij"] += T1["ai"] * T1["bj"];                        // :232
-        TauSA["abij"] = 2 * Tau["abij"];                           // :234
-        TauSA["abij"] -= Tau["abji"];                              // :235
-        T2SA["abij"] = 2 * T2["abij"];                             // :237
-        T2SA["abij"] -= T2["abji"];                                // :238
+        // :231-238 in one pass over T2, rounded as the six operator calls
+        //   Tau["abij"] = T2["abij"];  Tau["abij"] += T1["ai"] * T1["bj"];
+        //   TauSA["abij"] = 2 * Tau["abij"];  TauSA["abij"] -= Tau["abji"];
+        //   T2SA["abij"] = 2 * T2["abij"];  T2SA["abij"] -= T2["abji"];
+        launch_tau_sa(ctx, dptr(T1), dptr(T2), v, o, dptr(Tau), dptr(TauSA), dptr(T2SA));
+        bump_t(Tau);
+        bump_t(TauSA);
+        bump_t(T2SA);
 
         FME["me"] = fAI["em"];                                     // :244
         FME["me"] += VABIJSA["efmn"] * T1["fn"];                   // :245

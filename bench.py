@@ -300,6 +300,67 @@ def copy_calibration(n=1 << 27, reps=3):
                                            "launches": reps, "timing": tim}}))
 
 
+def hbm_ceiling(v=288, o=48, reps=5):
+    """Achievable HBM bandwidth next to the library's HBM-bound kernels on
+    amplitude-sized FP64 tensors [v,v,o,o] (o=48, v=288: 1.53 GB each): a
+    plain device copy through torch, then the permuted copies and adds,
+    transposes, dot and the fused amplitude update as the solver issues them.
+    Bytes are algorithmic (8 B per element read or written)."""
+    import torch
+    import aquarius_amd as aq
+    out = []
+    n = v * v * o * o
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    out.append({"kernel": "torch Tensor.copy_", "what": "y = x", "ms": ms, "GBps": 16.0 * n / ms / 1e6})
+    del x, y
+    torch.cuda.empty_cache()
+    with aq.Context(0) as ctx:
+        A = ctx.tensor((v, v, o, o), zero=False)
+        A2 = ctx.tensor((v, o, v, o), zero=False)
+        B = ctx.tensor((v, v, o, o), zero=False)
+        for t in (A, A2, B):
+            ctx.fill(t, 0.5, 0.0)
+        cases = [(A, "abij", 0.0, "copy"), (A, "abij", 1.0, "add"), (A, "abji", 0.0, "copy, i<->j"),
+                 (A, "abji", 1.0, "add, i<->j"), (A, "baji", 1.0, "add, a<->b and i<->j"),
+                 (A2, "aibj", 0.0, "copy, LDS transpose"), (A2, "aibj", 1.0, "add, LDS transpose")]
+        for src, ia, beta, what in cases:
+            ctx.sum(1.0, src, ia, beta, B, "abij")
+            ctx.sync()
+            ctx.enable_timing(True)
+            ctx.timing_reset()
+            for _ in range(reps):
+                ctx.sum(1.0, src, ia, beta, B, "abij")
+            ctx.sync()
+            for k, t in ctx.timing().items():
+                if t["launches"]:
+                    ms = t["total_ms"] / t["launches"]
+                    out.append({"kernel": k, "what": f"B[abij] = {'B + ' if beta else ''}A[{ia}] ({what})",
+                                "ms": ms, "GBps": t["bytes"] / t["launches"] / ms / 1e6})
+            ctx.enable_timing(False)
+        ctx.enable_timing(True)
+        ctx.timing_reset()
+        for _ in range(reps):
+            ctx.dot(A, "abij", B, "abij")
+        for k, t in ctx.timing().items():
+            if t["launches"] and k == "dot":
+                ms = t["total_ms"] / t["launches"]
+                out.append({"kernel": k, "what": "sum A*B", "ms": ms, "GBps": t["bytes"] / t["launches"] / ms / 1e6})
+        ctx.enable_timing(False)
+    for r in out:
+        print(json.dumps(r))
+
+
 def only(a):
     """W warmup + exactly K timed iterations of one solve (FP64, or the mixed
     policy from its first iteration) and nothing else: the command profiled
@@ -504,6 +565,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solve")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="profiling run: W warmup + K timed iterations of one solve, then exit")
+    ap.add_argument("--hbm-ceiling", action="store_true",
+                    help="time a plain device copy beside the permute/transpose/dot kernels (HBM bandwidth ceiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="profiling run: plain device copies of a 1 GiB FP64 tensor (HBM counter calibration)")
     ap.add_argument("--autotune", type=int, default=1, help="GEMM autotuner on (1) or off (0)")
@@ -535,6 +598,8 @@ def main():
         return
     if a.copy_calibration:
         return copy_calibration()
+    if a.hbm_ceiling:
+        return hbm_ceiling()
     if a.only:
         return only(a)
 
