@@ -732,24 +732,77 @@ __global__ __launch_bounds__(TPB) void k_tau_sa(const double* __restrict__ T1, c
     }
 }
 
-// Z[v,v,o,o] += alpha * (S+ + S-) expanded to all (a,b,i,j)
+// Z[v,v,o,o] += alpha * (S+ + S-) expanded to all (a,b,i,j):
+//   Z(ab,ij) += S+ + S-,  Z(ba,ij) += S+ - S-,  Z(ab,ji) += S+ - S-,  Z(ba,ji) += S+ + S-
+// for a < b, i < j (S- vanishes when a = b or i = j).  A block takes a 32 x 32
+// tile of (a,b) pairs with a-tile <= b-tile and one pair i <= j: it reads the
+// tile's packed values along a (contiguous in S+- for a <= b), keeps them in
+// LDS and writes the (a,b) tile along a and its mirror (b,a) along b, for
+// both slabs (i,j) and (j,i).  Every Z element is written by one block.  A
+// diagonal tile (a-tile = b-tile) writes only its own (a,b) positions, reading
+// the packed value of (min, max).
+constexpr int UP_T = 32;
 __global__ __launch_bounds__(TPB) void k_ladder_unpack(const double* __restrict__ Sp, const double* __restrict__ Sm,
                                                        int64_t v, int64_t o, double alpha, double* __restrict__ Z) {
+    static_assert(TPB % UP_T == 0, "whole tile rows per pass");
+    constexpr int R = TPB / UP_T;
+    __shared__ double lp[UP_T][UP_T + 1], lm[UP_T][UP_T + 1];
     const int64_t P = v * (v + 1) / 2, Q = v * (v - 1) / 2;
-    const int64_t vv = v * v;
-    for (int64_t c = blockIdx.y; c < o * o; c += gridDim.y) {
-        const int64_t i = c % o, j = c / o;
-        const int64_t ilo = i < j ? i : j, ihi = i < j ? j : i;
-        const double sij = i < j ? 1.0 : -1.0;
-        const double* Spc = Sp + tri_le(ilo, ihi) * P;
-        const double* Smc = i != j ? Sm + tri_lt(ilo, ihi) * Q : nullptr;
-        double* Zc = Z + c * vv;
-        for (int64_t r = int64_t(blockIdx.x) * TPB + threadIdx.x; r < vv; r += int64_t(gridDim.x) * TPB) {
-            const int64_t a = r % v, b = r / v;
-            const int64_t alo = a < b ? a : b, ahi = a < b ? b : a;
-            double s = Spc[tri_le(alo, ahi)];
-            if (Smc && a != b) s += (a < b ? sij : -sij) * Smc[tri_lt(alo, ahi)];
-            Zc[r] += alpha * s;
+    const int64_t vv = v * v, np = o * (o + 1) / 2;
+    // upper tile pair (ta <= tb) of this block
+    int64_t ta, tb;
+    untri_le(blockIdx.x, ta, tb);
+    const int64_t a0 = ta * UP_T, b0 = tb * UP_T;
+    const bool diag = ta == tb;
+    const int tx = threadIdx.x % UP_T, ty = threadIdx.x / UP_T;
+    for (int64_t c = blockIdx.y; c < np; c += gridDim.y) {
+        int64_t i, j;
+        untri_le(c, i, j);
+        const double* Spc = Sp + c * P;                                       // tri_le(i,j) = c
+        const double* Smc = i < j ? Sm + (c - j) * Q : nullptr;               // tri_lt(i,j) = c - j
+        __syncthreads();   // the previous pair's tile has been written out
+        // load: lp/lm[al][bl] for the tile's (a,b); a fastest
+#pragma unroll
+        for (int r = 0; r < UP_T; r += R) {
+            const int al = tx, bl = ty + r;
+            const int64_t a = a0 + al, b = b0 + bl;
+            double sp = 0.0, sm = 0.0;
+            if (a < v && b < v && (!diag || al <= bl)) {
+                sp = Spc[tri_le(a, b)];
+                if (Smc && a < b) sm = Smc[tri_lt(a, b)];
+            } else if (diag && a < v && b < v) {
+                sp = Spc[tri_le(b, a)];
+                if (Smc) sm = -Smc[tri_lt(b, a)];                             // S-(ab) = -S-(ba)
+            }
+            lp[al][bl] = sp;
+            lm[al][bl] = sm;
+        }
+        __syncthreads();
+        double* Zij = Z + (i + j * o) * vv;
+        double* Zji = Z + (j + i * o) * vv;
+        // (a,b) positions, a fastest: Z(ab,ij) += S+ + S-, Z(ab,ji) += S+ - S-
+#pragma unroll
+        for (int r = 0; r < UP_T; r += R) {
+            const int al = tx, bl = ty + r;
+            const int64_t a = a0 + al, b = b0 + bl;
+            if (a < v && b < v) {
+                const double sp = lp[al][bl], sm = lm[al][bl];
+                Zij[a + b * v] += alpha * (sp + sm);
+                if (i != j) Zji[a + b * v] += alpha * (sp - sm);
+            }
+        }
+        if (!diag) {
+            // mirror (b,a) positions, b fastest: Z(ba,ij) += S+ - S-, Z(ba,ji) += S+ + S-
+#pragma unroll
+            for (int r = 0; r < UP_T; r += R) {
+                const int bl = tx, al = ty + r;
+                const int64_t a = a0 + al, b = b0 + bl;
+                if (a < v && b < v) {
+                    const double sp = lp[al][bl], sm = lm[al][bl];
+                    Zij[b + a * v] += alpha * (sp - sm);
+                    if (i != j) Zji[b + a * v] += alpha * (sp + sm);
+                }
+            }
         }
     }
 }
@@ -804,10 +857,12 @@ void launch_tau_sa(aq_ctx* ctx, const double* T1, const double* T2, int v, int o
 void launch_ladder_unpack(aq_ctx* ctx, const double* Sp, const double* Sm, int v, int o, double alpha, double* Z) {
     const int64_t vv = int64_t(v) * v, oo = int64_t(o) * o;
     TimedLaunch tl(ctx, "ladder_unpack", 0.0, 8.0 * 3.0 * vv * oo);
-    const unsigned gx = unsigned(std::min<int64_t>((vv + TPB - 1) / TPB, 64));
-    const unsigned gy = unsigned(std::min<int64_t>(oo, 65535));
-    hipLaunchKernelGGL(k_ladder_unpack, dim3(gx, gy), dim3(TPB), 0, ctx->stream, Sp, Sm, int64_t(v), int64_t(o),
-                       alpha, Z);
+    const int64_t nt = (int64_t(v) + UP_T - 1) / UP_T;
+    const int64_t gx = nt * (nt + 1) / 2, np = int64_t(o) * (o + 1) / 2;
+    AQ_CHECK(gx < (int64_t(1) << 31), AQ_ERR_UNSUPPORTED, "ladder unpack grid too large");
+    const unsigned gy = unsigned(std::min<int64_t>(np, 65535));
+    hipLaunchKernelGGL(k_ladder_unpack, dim3(unsigned(gx), gy), dim3(TPB), 0, ctx->stream, Sp, Sm, int64_t(v),
+                       int64_t(o), alpha, Z);
     AQ_HIP(hipGetLastError());
 }
 

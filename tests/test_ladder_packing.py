@@ -102,6 +102,55 @@ def unpack(Sp, Sm, v, o, alpha, Z):
     return Zf.reshape(Z.shape, order="F")
 
 
+def unpack_tiled(Sp, Sm, v, o, alpha, Z, T=4):
+    """The device kernel's schedule: upper (a-tile <= b-tile) tiles of T x T
+    (a,b) pairs and one pair i <= j per block; the (a,b) tile and its mirror
+    (b,a), slabs (i,j) and (j,i).  Records that every Z element is updated
+    exactly once and every read stays inside S+ / S-."""
+    P, Q = v * (v + 1) // 2, v * (v - 1) // 2
+    Spf, Smf = Sp.reshape(-1, order="F"), Sm.reshape(-1, order="F")
+    Zf = Z.reshape(-1, order="F").copy()
+    hits = np.zeros(Zf.size, dtype=int)
+    vv, nt = v * v, (v + T - 1) // T
+    for bx in range(nt * (nt + 1) // 2):
+        ta, tb = untri_le(bx)
+        diag = ta == tb
+        for c in range(o * (o + 1) // 2):
+            i, j = untri_le(c)
+            lp, lm = np.zeros((T, T)), np.zeros((T, T))
+            for al in range(T):
+                for bl in range(T):
+                    a, b = ta * T + al, tb * T + bl
+                    if a >= v or b >= v:
+                        continue
+                    lo, hi = (a, b) if (not diag or al <= bl) else (b, a)
+                    k = tri_le(lo, hi) + c * P
+                    assert 0 <= k < Spf.size
+                    lp[al, bl] = Spf[k]
+                    if i < j and lo < hi:
+                        k = tri_lt(lo, hi) + (c - j) * Q
+                        assert c - j == tri_lt(i, j) and 0 <= k < Smf.size
+                        lm[al, bl] = Smf[k] if lo == a else -Smf[k]
+            for al in range(T):
+                for bl in range(T):
+                    a, b = ta * T + al, tb * T + bl
+                    if a >= v or b >= v:
+                        continue
+                    sp, sm = lp[al, bl], lm[al, bl]
+                    writes = [(a + b * v + (i + j * o) * vv, sp + sm)]
+                    if i != j:
+                        writes.append((a + b * v + (j + i * o) * vv, sp - sm))
+                    if not diag:
+                        writes.append((b + a * v + (i + j * o) * vv, sp - sm))
+                        if i != j:
+                            writes.append((b + a * v + (j + i * o) * vv, sp + sm))
+                    for k, s in writes:
+                        Zf[k] += alpha * s
+                        hits[k] += 1
+    assert (hits == 1).all()
+    return Zf.reshape(Z.shape, order="F")
+
+
 def symmetric_inputs(rng, v, o):
     V = rng.standard_normal((v, v, v, v))
     V = V + V.transpose(1, 0, 3, 2)                  # V(ab,ef) = V(ba,fe)
@@ -122,6 +171,9 @@ def test_packed_ladder_equals_full_contraction(v, o):
     Z = unpack(Vp @ Tp.T, Vm @ Tm.T, v, o, 0.5, Z0)
     ref = Z0 + 0.5 * np.einsum("abef,efij->abij", V, Tau)
     np.testing.assert_allclose(Z, ref, rtol=1e-12, atol=1e-12)
+    # the tiled schedule of the device kernel (ragged last tile when T does not divide v)
+    for T in (2, 3, 4):
+        np.testing.assert_array_equal(unpack_tiled(Vp @ Tp.T, Vm @ Tm.T, v, o, 0.5, Z0, T), Z)
 
 
 def test_packed_ladder_flop_ratio():
