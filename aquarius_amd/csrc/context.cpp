@@ -7,6 +7,9 @@
 #include <atomic>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace aqamd {
@@ -79,6 +82,15 @@ void invalidate_cache(aq_ctx* ctx) {
     for (auto& kv : ctx->cache) kv.second.valid = false;
 }
 
+// AQ_ALLOC_TRACE=1: one line on stderr per allocation that needed the retry path
+static bool alloc_trace() {
+    static const bool on = [] {
+        const char* e = getenv("AQ_ALLOC_TRACE");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+
 static void* raw_alloc(aq_ctx* ctx, size_t bytes) {
     // under graph capture an allocation would become part of the graph;
     // the capturing caller abandons the capture and runs eagerly instead
@@ -92,11 +104,23 @@ static void* raw_alloc(aq_ctx* ctx, size_t bytes) {
                  std::to_string(ctx_in_use(ctx)) + " of " + std::to_string(ctx->budget));
     void* p = dmalloc(ctx, bytes);
     if (!p) {
+        // The device pool holds freed blocks of other sizes (and other contexts'
+        // memory): hand the unused blocks back to the driver and retry before
+        // giving up any cached operand copy.
+        const auto t0 = std::chrono::steady_clock::now();
         pool_release_all(ctx);
-        evict_cache(ctx, ctx->budget);   // drop every cached copy
         AQ_HIP(hipStreamSynchronize(ctx->stream));
         trim_device_pool(ctx);
         p = dmalloc(ctx, bytes);
+        if (!p) {
+            evict_cache(ctx, ctx->budget);   // drop every cached copy
+            AQ_HIP(hipStreamSynchronize(ctx->stream));
+            trim_device_pool(ctx);
+            p = dmalloc(ctx, bytes);
+        }
+        if (alloc_trace())
+            fprintf(stderr, "aq_alloc retry: %zu B, %s, %.1f ms\n", bytes, p ? "served" : "failed",
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
         if (!p) throw Error(AQ_ERR_OOM, "device allocation of " + std::to_string(bytes) + " B failed");
     }
     return p;

@@ -434,24 +434,27 @@ void launch_scatter_keys(aq_ctx* ctx, double* x, int64_t n, const int64_t* keys,
 // -------------------------------------------------------------- split-K
 
 __global__ __launch_bounds__(TPB) void k_splitk(const double* __restrict__ P, int nsplit, int64_t m, int64_t n,
-                                                 int64_t batch, double beta, double* C, int64_t ldc, int64_t strideC) {
+                                                 int64_t batch, double beta, double* C, int64_t ldc, int64_t strideC,
+                                                 CMap map) {
     const int64_t mn = m * n, tot = mn * batch;
     for (int64_t i = int64_t(blockIdx.x) * TPB + threadIdx.x; i < tot; i += int64_t(gridDim.x) * TPB) {
         double s = 0.0;
         for (int k = 0; k < nsplit; ++k) s += P[int64_t(k) * tot + i];
         const int64_t b = i / mn, r = i % mn;
         const int64_t col = r / m, row = r % m;
-        double* c = C + b * strideC + row + col * ldc;
+        const int64_t off = map.mapped ? cmap_digits(map.nm, map.ml, map.ms, row) + cmap_digits(map.nn, map.nl, map.ns, col)
+                                       : row + col * ldc;
+        double* c = C + b * strideC + off;
         *c = beta == 0.0 ? s : beta * *c + s;
     }
 }
 
 void launch_reduce_splitk(aq_ctx* ctx, const double* P, int nsplit, int64_t m, int64_t n, int64_t batch, double beta,
-                          double* C, int64_t ldc, int64_t strideC) {
+                          double* C, int64_t ldc, int64_t strideC, const CMap& map) {
     const int64_t tot = m * n * batch;
     TimedLaunch tl(ctx, "splitk_reduce", double(tot) * nsplit, 8.0 * tot * (nsplit + 1));
     hipLaunchKernelGGL(k_splitk, dim3(grid_for(tot, TPB)), dim3(TPB), 0, ctx->stream, P, nsplit, m, n, batch, beta, C,
-                       ldc, strideC);
+                       ldc, strideC, map);
     AQ_HIP(hipGetLastError());
 }
 

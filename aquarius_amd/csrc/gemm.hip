@@ -60,7 +60,22 @@ struct GemmArgs {
     int transA, transB;
     int off32;   // every per-thread tile offset of A and B fits 32 bits (FP64/FP32 interior fast path)
     int raster;  // tile order: 0 = rows of tiles; G > 0 = groups of G tile rows (tile_of)
+    CMap cmap;   // placement of the output in C (not used for split-K partials)
 };
+
+// C offsets of one lane's output columns n (FN blocks x 4 registers), as
+// column-major C (n * ldc) or through the output map; computed once per tile.
+template <int FN, class ColOf>
+__device__ __forceinline__ void column_offsets(const GemmArgs& g, bool mapped, int64_t ldc, ColOf col,
+                                               int64_t (&co)[FN][4]) {
+#pragma unroll
+    for (int j = 0; j < FN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t n = col(j, r);
+            co[j][r] = mapped ? (n < g.N ? cmap_digits(g.cmap.nn, g.cmap.nl, g.cmap.ns, n) : 0) : n * ldc;
+        }
+}
 
 // tile t -> (tile row, tile col).  G == 0: rows of tiles, n fastest.  G > 0:
 // groups of G tile rows, m fastest within a group, so a run of consecutive
@@ -392,6 +407,10 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
     // epilogue: lane holds C[m = mb + lane%16][n = nb + drow(lane, r)]
     double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
     const int64_t ldc = g.P ? g.M : g.ldc;
+    const bool mapped = !g.P && g.cmap.mapped;
+    int64_t co[FN][4];
+    column_offsets<FN>(g, mapped, ldc,
+                       [&](int j, int r) { return n0 + wn * WTN + j * 16 + Mfma<T>::drow(lane, r); }, co);
     // per row of blocks, every C value is read before any is written: one
     // memory round trip per row instead of one per element (a load after a
     // store to C could alias it, so the compiler keeps that order)
@@ -399,6 +418,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int64_t m = m0 + wm * WTM + i * 16 + lane % 16;
+        const int64_t ro = mapped ? (m < g.M ? cmap_digits(g.cmap.nm, g.cmap.ml, g.cmap.ms, m) : 0) : m;
         double cv[FN][4];
         if (addc) {
 #pragma unroll
@@ -406,7 +426,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int64_t n = n0 + wn * WTN + j * 16 + Mfma<T>::drow(lane, r);
-                    cv[j][r] = (m < g.M && n < g.N) ? Cb[m + n * ldc] : 0.0;
+                    cv[j][r] = (m < g.M && n < g.N) ? Cb[ro + co[j][r]] : 0.0;
                 }
         }
 #pragma unroll
@@ -417,7 +437,7 @@ __device__ __forceinline__ void gemm_tile(const GemmArgs g, int64_t bx, int bz) 
                 if (m < g.M && n < g.N) {
                     double v = g.alpha * double(acc[i][j][r]);
                     if (addc) v += g.beta * cv[j][r];
-                    Cb[m + n * ldc] = v;
+                    Cb[ro + co[j][r]] = v;
                 }
             }
         }
@@ -698,12 +718,17 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
 
     double* Cb = g.P ? g.P + (int64_t(split) * g.batch + b) * (g.M * g.N) : g.C + int64_t(b) * g.strideC;
     const int64_t ldc = g.P ? g.M : g.ldc;
+    const bool mapped = !g.P && g.cmap.mapped;
+    int64_t co[FN][4];
+    column_offsets<FN>(g, mapped, ldc, [&](int j, int r) { return n0 + wn * WTN + j * 16 + 4 * (lane / 16) + r; },
+                       co);
     // per row of blocks, every C value is read before any is written (one
     // memory round trip per row instead of one per element)
     const bool addc = !g.P && g.beta != 0.0;
 #pragma unroll
     for (int i = 0; i < FM; ++i) {
         const int64_t m = m0 + wm * WTM + i * 16 + lane % 16;
+        const int64_t ro = mapped ? (m < g.M ? cmap_digits(g.cmap.nm, g.cmap.ml, g.cmap.ms, m) : 0) : m;
         double cv[FN][4];
         if (addc) {
 #pragma unroll
@@ -711,7 +736,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int64_t n = n0 + wn * WTN + j * 16 + 4 * (lane / 16) + r;
-                    cv[j][r] = (m < g.M && n < g.N) ? Cb[m + n * ldc] : 0.0;
+                    cv[j][r] = (m < g.M && n < g.N) ? Cb[ro + co[j][r]] : 0.0;
                 }
         }
 #pragma unroll
@@ -722,7 +747,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
                 if (m < g.M && n < g.N) {
                     double v = g.alpha * double(acc[i][j][r]);
                     if (addc) v += g.beta * cv[j][r];
-                    Cb[m + n * ldc] = v;
+                    Cb[ro + co[j][r]] = v;
                 }
             }
         }
@@ -916,6 +941,7 @@ static int tuned_variant(aq_ctx* ctx, const GemmDesc& d, int BM, int BN, const G
         t.strideC = a.M * a.N;
         t.beta = 0.0;
         t.P = a.nsplit > 1 ? tp.as<double>() : nullptr;
+        t.cmap = CMap{};   // the trial writes its scratch output column-major
         hipEvent_t e0, e1;
         AQ_HIP(hipEventCreate(&e0));
         AQ_HIP(hipEventCreate(&e1));
@@ -1245,6 +1271,9 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     a.batch = int(d.batch);
     a.transA = d.transA; a.transB = d.transB;
     a.P = nullptr;
+    a.cmap = d.cmap;
+    AQ_CHECK(!d.cmap.mapped || (d.M < (int64_t(1) << 31) && d.N < (int64_t(1) << 31)), AQ_ERR_INVALID,
+             "mapped GEMM output needs M and N below 2^31");
     // an element of a tile lies below 144 * ld + 256 elements from the tile origin
     a.off32 = (144 * std::max(d.lda, d.ldb) + 256) * 8 < (int64_t(1) << 31) ? 1 : 0;
     // groups of 8 tile rows: measured on the 13824^3 ring GEMMs, BF16x3 279 -> 357 TFLOP/s
@@ -1260,7 +1289,7 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
         if (batcher_submit(ctx, GroupKey{int(d.fmt), d.transA ? 1 : 0, d.transB ? 1 : 0, BM, BN}, a,
                            base_tiles * d.batch * nsplit)) {
             if (nsplit > 1)
-                launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
+                launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC, d.cmap);
             return;
         }
         a.P = nullptr;   // alone: the launch below allocates its own partials
@@ -1300,7 +1329,7 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
         AQ_HIP(hipGetLastError());
     }
     if (nsplit > 1)
-        launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC);
+        launch_reduce_splitk(ctx, a.P, nsplit, d.M, d.N, d.batch, d.beta, d.C, d.ldc, d.strideC, d.cmap);
 }
 
 int autotune_entries(aq_tune_entry* out, int cap) {

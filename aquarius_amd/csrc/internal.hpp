@@ -232,8 +232,31 @@ double sync_result(aq_ctx* ctx, int slot);
 void launch_scatter_keys(aq_ctx* ctx, double* x, int64_t n, const int64_t* keys, const double* vals,
                          double alpha, double beta);
 void launch_gather_keys(aq_ctx* ctx, const double* x, int64_t n, const int64_t* keys, double* vals);
+// Placement of a GEMM's [M x N] output in C.  Unmapped: C[m + n*ldc].  Mapped:
+// C[rows(m) + cols(n)] with m = d0 + ml[0]*(d1 + ml[1]*d2) (digits of the GEMM
+// row labels, first fastest) and rows(m) = sum dk*ms[k]; the same for n.  Lets a
+// contraction write its result straight into a permuted C (no temporary and no
+// permuting pass).  M and N below 2^31.
+struct CMap {
+    int mapped = 0;
+    int nm = 0, nn = 0;
+    int64_t ml[3] = {1, 1, 1}, ms[3] = {0, 0, 0};
+    int64_t nl[3] = {1, 1, 1}, ns[3] = {0, 0, 0};
+};
+// offset of a GEMM row (or column) index x < 2^31 through nd mapped digits
+__device__ __forceinline__ int64_t cmap_digits(int nd, const int64_t* len, const int64_t* st, int64_t x) {
+    uint32_t r = uint32_t(x);
+    int64_t off = 0;
+    for (int k = 0; k + 1 < nd; ++k) {
+        const uint32_t l = uint32_t(len[k]);
+        const uint32_t q = r / l;
+        off += int64_t(r - q * l) * st[k];
+        r = q;
+    }
+    return nd > 0 ? off + int64_t(r) * st[nd - 1] : 0;
+}
 void launch_reduce_splitk(aq_ctx* ctx, const double* P, int nsplit, int64_t m, int64_t n,
-                          int64_t batch, double beta, double* C, int64_t ldc, int64_t strideC);
+                          int64_t batch, double beta, double* C, int64_t ldc, int64_t strideC, const CMap& map);
 
 // fused CCSD update (elementwise.hip)
 struct UpdateArgs {
@@ -278,6 +301,7 @@ struct GemmDesc {
     double alpha, beta;
     double* C; int64_t ldc; int64_t strideC;
     const char* tag;   // timing class name
+    CMap cmap;         // output placement (unmapped: C[m + n*ldc] + b*strideC)
 };
 int autotune_entries(aq_tune_entry* out, int cap);   // gemm.hip
 void gemm(aq_ctx* ctx, const GemmDesc& d);

@@ -406,19 +406,41 @@ static void contract_impl(aq_ctx* ctx, double alpha, const aq_tensor* tA, const 
     struct Choice {
         double cost;
         std::string Mo, No, Ko;
-        bool swap, cdirect, adirect, atrans, bdirect, btrans;
-    } best{1e300, "", "", "", false, false, false, false, false, false};
+        bool swap, cdirect, adirect, atrans, bdirect, btrans, cmapped;
+    } best{1e300, "", "", "", false, false, false, false, false, false, false};
+
+    // Can the GEMM output [rows cols Bo] be written straight into C through a
+    // CMap?  At most three non-unit labels per side, at most one batch label,
+    // and the first non-unit row label contiguous in C (a wavefront's 16
+    // consecutive rows then write one contiguous run).
+    auto non_unit = [&](const std::string& labs) {
+        std::string r;
+        for (char c : labs)
+            if (C0.len[C0.lab.find(c)] != 1) r.push_back(c);
+        return r;
+    };
+    auto mappable = [&](const std::string& rows, const std::string& cols) {
+        const std::string r = non_unit(rows), c = non_unit(cols), b = non_unit(Bo);
+        if (r.empty() || r.size() > 3 || c.size() > 3 || b.size() > 1) return false;
+        if (C0.stride[C0.lab.find(r[0])] != 1) return false;
+        return prod_len(rows, C0) < (int64_t(1) << 31) && prod_len(cols, C0) < (int64_t(1) << 31);
+    };
 
     const double szA = double(A.size()), szB = double(B.size()), szC = double(C0.size());
     for (auto& Mo : Mc)
         for (auto& No : Nc)
             for (auto& Ko : Kc)
                 for (int sw = 0; sw < 2; ++sw) {
-                    Choice ch{0.0, Mo, No, Ko, sw == 1, false, false, false, false, false};
+                    Choice ch{0.0, Mo, No, Ko, sw == 1, false, false, false, false, false, false};
                     // GEMM output layout: [Mg Ng Bo]
                     const std::string outl = sw ? No + Mo + Bo : Mo + No + Bo;
                     ch.cdirect = c_simple && is_view(C0, outl);
-                    if (!ch.cdirect) ch.cost += 3.0 * szC;   // temp write + read + C read/write
+                    if (!ch.cdirect) {
+                        ch.cmapped = c_simple && (sw ? mappable(No, Mo) : mappable(Mo, No));
+                        // mapped: the epilogue reads and writes C in place (a little index
+                        // arithmetic); otherwise temp write + read + C read/write
+                        ch.cost += (ch.cmapped ? 0.25 : 3.0) * szC;
+                    }
                     // GEMM-A is A (rows Mo) or, swapped, B (rows No)
                     auto check = [&](const Labeled& L, const std::string& rows, bool& direct, bool& trans,
                                      double sz) {
@@ -500,6 +522,29 @@ static void contract_impl(aq_ctx* ctx, double alpha, const aq_tensor* tA, const 
         d.C = tC->data;
         d.ldc = Mg;
         d.strideC = Mg * Ng;
+        d.beta = beta;
+        gemm(ctx, d);
+    } else if (ch.cmapped) {
+        // the output [Rg Cg Bo] goes straight into the permuted C
+        auto digits = [&](const std::string& labs, int& nd, int64_t* len, int64_t* st) {
+            nd = 0;
+            for (char c : labs) {
+                const size_t k = C0.lab.find(c);
+                if (C0.len[k] == 1) continue;
+                len[nd] = C0.len[k];
+                st[nd] = C0.stride[k];
+                ++nd;
+            }
+        };
+        d.cmap.mapped = 1;
+        digits(Rg, d.cmap.nm, d.cmap.ml, d.cmap.ms);
+        digits(Cg, d.cmap.nn, d.cmap.nl, d.cmap.ns);
+        int nb = 0;
+        int64_t blen[3] = {1, 1, 1}, bst[3] = {0, 0, 0};
+        digits(Bo, nb, blen, bst);
+        d.C = tC->data;
+        d.ldc = Mg;
+        d.strideC = nb ? bst[0] : 0;
         d.beta = beta;
         gemm(ctx, d);
     } else {

@@ -93,6 +93,32 @@ def test_contract_one_short_side_tiles(ctx, prec, tol, ia, ib):
     check_contract(ctx, ia, ib, "mn", {"m": 40, "n": 33001, "k": 37}, seed=32, tol=tol, prec=prec)
 
 
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("bf16x3", 5e-5)])
+@pytest.mark.parametrize("ia,ib,ic,dim", [
+    # ring: GEMM rows (a,i), columns (b,j) written into C[abij] through the output map
+    ("amie", "ebmj", "abij", {"a": 30, "b": 30, "e": 30, "m": 8, "i": 8, "j": 8}),
+    # a batch label z beside two row labels
+    ("aekz", "ebz", "abkz", {"a": 40, "b": 24, "e": 32, "k": 6, "z": 3}),
+    # rows (a,m) around the column b in C; 7 columns and K = 3000: split-K partials reduced through the map
+    ("akm", "kb", "abm", {"a": 20, "k": 3000, "m": 5, "b": 7}),
+    # three row digits
+    ("aeij", "eb", "aibj", {"a": 36, "e": 40, "i": 5, "j": 6, "b": 9}),
+])
+def test_contract_mapped_output(ctx, prec, tol, ia, ib, ic, dim):
+    """Contractions whose C is a permutation of the GEMM output [rows cols
+    batch]: the GEMM epilogue (or the split-K reduction) writes C in place
+    through per-label strides, with no temporary and no permuting pass."""
+    print(f"case: mapped output {prec} {ia},{ib}->{ic}", flush=True)
+    ctx.enable_timing(True)
+    ctx.timing_reset()
+    try:
+        check_contract(ctx, ia, ib, ic, dim, beta=0.5, seed=41, tol=tol, prec=prec)
+        tim = ctx.timing()
+    finally:
+        ctx.enable_timing(False)
+    assert tim.get("strided_sum", {"launches": 0})["launches"] == 0, tim
+
+
 def test_contract_batch_and_special_labels(ctx):
     dim = {"a": 7, "b": 5, "k": 9, "z": 3, "x": 4}
     check_contract(ctx, "akz", "kbz", "abz", dim, seed=4)          # batch label z
