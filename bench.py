@@ -362,6 +362,31 @@ def hbm_ceiling(v=288, o=48, reps=5):
         print(json.dumps(r))
 
 
+def trace_split(path, kernel="k_gemm<double, 128, 128, 2, 2, false, true, 16, 2>"):
+    """Per-solve launch statistics of one kernel in a rocprofv3 kernel trace
+    (run_kernel_trace.csv) of the default bench command: the trace is cut at
+    the solver setups, each of which launches k_pair_asym twice (warm-up FP64,
+    warm-up BF16x3, mixed solve, FP64 solve, then the later legs)."""
+    import csv
+    setups, launches = [], []
+    with open(path) as fh:
+        for r in csv.DictReader(fh):
+            name = r["Kernel_Name"]
+            if "k_pair_asym" in name:
+                setups.append(int(r["Start_Timestamp"]))
+            elif kernel in name:
+                t0, t1 = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+                launches.append((t0, (t1 - t0) / 1e6))
+    setups.sort()
+    launches.sort()
+    cuts = setups[0:8:2] + [setups[8] if len(setups) > 8 else 1 << 62]
+    out = {}
+    for label, lo, hi in zip(("warm-up fp64", "warm-up bf16x3", "mixed solve", "fp64 solve"), cuts[:-1], cuts[1:]):
+        d = [ms for t, ms in launches if lo <= t < hi]
+        out[label] = {"launches": len(d), "mean_ms": sum(d) / len(d) if d else None}
+    print(json.dumps({"kernel": kernel, "per_solve": out}))
+
+
 def only(a):
     """W warmup + exactly K timed iterations of one solve (FP64, or the mixed
     policy from its first iteration) and nothing else: the command profiled
@@ -566,6 +591,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 reference solve")
     ap.add_argument("--only", choices=["fp64", "mixed"], default=None,
                     help="profiling run: W warmup + K timed iterations of one solve, then exit")
+    ap.add_argument("--trace-split", default=None, metavar="RUN_KERNEL_TRACE_CSV",
+                    help="per-solve launch statistics of the dominant kernel in a rocprofv3 trace of this bench")
     ap.add_argument("--hbm-ceiling", action="store_true",
                     help="time a plain device copy beside the permute/transpose/dot kernels (HBM bandwidth ceiling)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -601,6 +628,8 @@ def main():
         return copy_calibration()
     if a.hbm_ceiling:
         return hbm_ceiling()
+    if a.trace_split:
+        return trace_split(a.trace_split)
     if a.only:
         return only(a)
 
