@@ -1,6 +1,6 @@
 """bench.py's host logic (no GPU): the FLOP model against the oracle's own
 contractions, the packed-ladder FLOP count, the serving job stream, and the
-refusal of more than one GPU."""
+refusal of more than one GPU, the per-solve split of a kernel trace."""
 
 import json
 import os
@@ -71,3 +71,33 @@ def test_more_than_one_gpu_is_refused():
     assert r.returncode == 2
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert line["n_gpus"] == 2 and "one GPU" in line["error"]
+
+
+def test_trace_split_cuts_at_solver_setups(tmp_path, capsys):
+    """bench.py --trace-split: the dominant kernel's launches are grouped by
+    the solver setups (two k_pair_asym launches each) of a kernel trace."""
+    nt = "void aqamd::k_gemm<double, 128, 128, 2, 2, false, true, 16, 2>(aqamd::GemmArgs)"
+    rows, t = [], 0
+
+    def add(name, ms):
+        nonlocal t
+        rows.append((name, t, t + int(ms * 1e6)))
+        t += int(ms * 1e6) + 1000
+
+    for seg, durations in enumerate(([1.0], [2.0, 2.0], [3.0, 5.0], [7.0])):
+        add("aqamd::k_pair_asym(double const*, long, long, double*)", 0.1)
+        add("aqamd::k_pair_asym(double const*, long, long, double*)", 0.1)
+        for ms in durations:
+            add(nt, ms)
+            add("void aqamd::k_strided<aqamd::SumOut>(...)", 0.5)
+    path = tmp_path / "run_kernel_trace.csv"
+    with open(path, "w") as fh:
+        fh.write('"Kind","Kernel_Name","Start_Timestamp","End_Timestamp"\n')
+        for name, a, b in rows:
+            fh.write(f'"KERNEL_DISPATCH","{name}",{a},{b}\n')
+    bench.trace_split(str(path))
+    out = json.loads(capsys.readouterr().out.strip().splitlines()[-1])["per_solve"]
+    assert out["warm-up fp64"] == {"launches": 1, "mean_ms": pytest.approx(1.0, rel=1e-6)}
+    assert out["warm-up bf16x3"]["launches"] == 2
+    assert out["mixed solve"] == {"launches": 2, "mean_ms": pytest.approx(4.0, rel=1e-6)}
+    assert out["fp64 solve"] == {"launches": 1, "mean_ms": pytest.approx(7.0, rel=1e-6)}
