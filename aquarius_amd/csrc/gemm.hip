@@ -573,7 +573,8 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
     // order, three products per block in the order lo*hi, hi*lo, hi*hi) is the
     // same on every path and in every variant.
     const int kq = 8 * (lane / 16);
-    auto mfma_tile = [&](const unsigned short (*as)[BM][KP], const unsigned short (*bs)[BN][KP], auto&& mid) {
+    auto mfma_tile = [&](const unsigned short (*as)[BM][KP], const unsigned short (*bs)[BN][KP], auto&& mid,
+                         auto&& late) {
         bf16x8 xh[FN], xl[FN], yh[2], yl[2];
         auto read_b = [&](int j) {
             const int nn = wn * WTN + j * 16 + lane % 16;
@@ -596,6 +597,9 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
             if (i + 1 < FM && i > 0) read_a(i + 1, (i + 1) & 1);
             __builtin_amdgcn_sched_barrier(0);   // keep those reads ahead of row i's MFMAs
             if (i == FM / 2) mid();
+            // every fragment of the last row is in registers by now: `late`
+            // may let other waves overwrite this tile's buffers
+            if (i == FM - 1 && FM > 1) late();
             if (i == 0) {
 #pragma unroll
                 for (int j = 0; j < FN; ++j) {
@@ -615,6 +619,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(xh[j], yh[i & 1], acc[i][j], 0, 0, 0);
             }
         }
+        if (FM == 1) late();
     };
 
     const int64_t nk = kend > kbeg ? (kend - kbeg + BK - 1) / BK : 0;
@@ -675,13 +680,17 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
                 }
             }
         };
+        // The barrier that publishes tile kt+1 comes before the last row of
+        // tile kt's MFMAs (whose fragments are already in registers), so the
+        // matrix core has that row's work across the barrier.
         auto step = [&](auto S, int64_t kt) {
             constexpr int s = decltype(S)::value;
             const int64_t k1 = kbeg + (kt + 1) * BK;
             if (kt + 1 < nk) issue(std::integral_constant<int, 1 - s>{}, k1);
-            mfma_tile(s ? As1 : As0, s ? Bs1 : Bs0, [] {});
-            if (kt + 1 < nk) land(std::integral_constant<int, 1 - s>{}, k1);
-            __syncthreads();
+            mfma_tile(s ? As1 : As0, s ? Bs1 : Bs0, [] {}, [&] {
+                if (kt + 1 < nk) land(std::integral_constant<int, 1 - s>{}, k1);
+                __syncthreads();
+            });
         };
         if (nk > 0) {
             issue(std::integral_constant<int, 0>{}, kbeg);
@@ -711,7 +720,7 @@ __device__ __forceinline__ void gemm_tile_bf16x3(const GemmArgs g, int64_t bx, i
                     store(buf ^ 1, ra0, rb0);
                     if (kt + 2 < nk) load(kbeg + (kt + 2) * BK, ra0, rb0);
                 }
-            });
+            }, [] {});
             __syncthreads();
         }
     }
