@@ -1250,9 +1250,11 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
         while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
     } else if (d.fmt == FMT_F64 && T > W && T <= 8 * W && ksteps >= 64) {
         // A grid of one to eight rounds with a ragged last round (the packed
-        // ladder: 3260 tiles = 6.4 rounds): split s ways when that cuts the
-        // rounds per unit of work, ceil(T*s / W) / s, by at least 5%, with at
-        // least 8 k-steps per split and at most 4 GiB of FP64 partials.
+        // ladder: 3260 and 2907 tiles = 6.4 and 5.7 rounds): split s ways when
+        // that cuts the rounds per unit of work, ceil(T*s / W) / s, by at
+        // least 4% (2907 tiles 7 ways: 4.8% on paper, 64.7 -> 59.6 ms
+        // measured), with at least 8 k-steps per split and at most 4 GiB of
+        // FP64 partials.
         // FP64 only: the BF16x3 phase keeps its operand copies in that memory.
         const double r1 = double((T + W - 1) / W);
         int64_t best = 1;
@@ -1265,7 +1267,7 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
                 best = sc;
             }
         }
-        if (best_cost <= 0.95 * r1) nsplit = int(best);
+        if (best_cost <= 0.96 * r1) nsplit = int(best);
         while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
     }
     int64_t kchunk = ((ksteps + nsplit - 1) / nsplit) * KSTEP;
