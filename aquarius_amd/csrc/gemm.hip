@@ -1269,6 +1269,29 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
         }
         if (best_cost <= 0.96 * r1) nsplit = int(best);
         while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
+    } else if (d.fmt == FMT_BF16X2 && d.M >= 256 && d.N >= 256 && ksteps >= 64) {
+        // BF16x3 GEMMs this large run on 256 x 256 blocks, one per CU: the
+        // same ragged-round rule over the grid of 256 x 256 tiles and W2 = 256
+        // (the packed ladder: 815 and 810 tiles = 3.2 rounds, split 5 ways
+        // fill 16 rounds instead of 4 rounds of whole tiles), with the partials
+        // capped at 2 GiB so that the operand cache keeps its copies.
+        const int64_t W2 = 256, T2 = ((d.M + 255) / 256) * ((d.N + 255) / 256) * d.batch;
+        if (T2 > W2 && T2 <= 8 * W2) {
+            const double r1 = double((T2 + W2 - 1) / W2);
+            int64_t best = 1;
+            double best_cost = r1;
+            for (int64_t sc = 2; sc <= 8 && ksteps / sc >= 8; ++sc) {
+                if (double(sc) * double(d.batch) * double(d.M) * double(d.N) * 8.0 > 2.0 * double(int64_t(1) << 30))
+                    break;
+                const double cost = double((T2 * sc + W2 - 1) / W2) / double(sc);
+                if (cost < best_cost - 1e-12) {
+                    best_cost = cost;
+                    best = sc;
+                }
+            }
+            if (best_cost <= 0.96 * r1) nsplit = int(best);
+            while (nsplit > 1 && int64_t(nsplit) * d.batch >= 65536) nsplit /= 2;
+        }
     }
     int64_t kchunk = ((ksteps + nsplit - 1) / nsplit) * KSTEP;
     nsplit = int((d.K + kchunk - 1) / kchunk);
