@@ -1235,7 +1235,11 @@ void gemm(aq_ctx* ctx, const GemmDesc& d) {
     int nsplit = 1;
     const int64_t ksteps = (d.K + KSTEP - 1) / KSTEP;
     const int64_t T = base_tiles * d.batch, W = 512;
+    // 64 x 64 blocks: four fit a CU, so a small grid is spread over up to
+    // ~two rounds of 1024
+    const int64_t Ws = (BM == 64 && BN == 64) ? 1024 : W;
     if (T < W && ksteps >= 16) {
+        const int64_t W = Ws;
         const int64_t smax = std::max<int64_t>(1, std::min<int64_t>({(2 * W + T - 1) / T, ksteps / 8, 256}));
         int64_t best = 1;
         double best_cost = 1.0;   // s = 1: one round per unit of work
