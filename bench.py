@@ -44,8 +44,8 @@ PEAK = {"fp64": 78.6e12, "fp32": 157.3e12, "bf16x3": 2.5e15}
 # gemm_f64_128x128_NT_o48v288_hbm_traffic.txt and bf16x3_iteration_o48v288.txt
 # (the BF16x3 class from `bench.py --only mixed --steps 1 --warmup 1`, the timed iteration, summarised
 # in bf16x3_iteration_o48v288.txt).
-TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 144.8e9,
-                      ("gemm_bf16x3_256x256_TN", 48, 288): 17.20e9}
+TRAFFIC_PER_LAUNCH = {("gemm_f64_128x128_NT", 48, 288): 135.6e9,
+                      ("gemm_bf16x3_256x256_TN", 48, 288): 17.39e9}
 
 
 def flops_per_iter(o, v):
